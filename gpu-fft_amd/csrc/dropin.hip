@@ -1,0 +1,202 @@
+// Drop-in entry points: GPU_FFT (C_C and R_R), GPU_FFNT and GPU_Special_FFT
+// of the reference (src/lib/fft.cu:517-725, 928-1007, 1386-1515), as C ABI.
+// Each call validates on the host, then enqueues one launch per pass on
+// cfg->stream (one launch in total up to 2^12 double / 2^13 float points).
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "status.hpp"
+#include "transform.hpp"
+
+using namespace gfa;
+
+namespace {
+
+int check_cfg(const gpufft_config* cfg, int lo, int hi, const char* who)
+{
+    if (!cfg) return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": cfg is NULL");
+    // same message as the reference (fft.cu:522-525)
+    if (cfg->n_power < lo || cfg->n_power > hi)
+        return fail(GPUFFT_ERR_INVALID_ARGUMENT, "Invalid n_power range!");
+    if (cfg->fft_type != GPUFFT_FORWARD && cfg->fft_type != GPUFFT_INVERSE)
+        return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": invalid fft_type");
+    return GPUFFT_OK;
+}
+
+int check_batch(int batch, const char* who)
+{
+    if (batch < 1) return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": batch must be >= 1");
+    return GPUFFT_OK;
+}
+
+template <typename T>
+int c2c(void* inout, const void* table, const gpufft_config* cfg, int batch, int mult,
+        const char* who)
+{
+    if (int s = check_cfg(cfg, GPUFFT_MIN_N_POWER, GPUFFT_MAX_N_POWER, who)) return s;
+    if (int s = check_batch(batch, who)) return s;
+    if (!inout || !table) return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": NULL buffer");
+    const int n = cfg->n_power;
+    const long long L = 1LL << n;
+    const int nega = cfg->reduction_poly == GPUFFT_X_N_PLUS;
+    hipStream_t st = (hipStream_t)cfg->stream;
+    cplx<T>* x = (cplx<T>*)inout;
+    TransformIO<T> io{};
+    io.src = x;
+    io.src_stride = L;
+    io.dst = x;
+    io.dst_stride = L;
+    io.work = x;
+    if (cfg->fft_type == GPUFFT_FORWARD) {
+        io.lk = LD_CPLX;
+        io.sk = ST_CPLX;
+        return run_transform<T>(MODE_STD_FWD, n, batch, nega, (const cplx<T>*)table, io, st, who);
+    }
+    io.lk = mult ? LD_MUL : LD_CPLX;
+    io.src2 = x + (long long)batch * L;
+    io.sk = ST_CPLX;
+    io.do_scale = true;
+    io.scale = mk<T>((T)cfg->mod_inverse_re, (T)cfg->mod_inverse_im);
+    return run_transform<T>(MODE_STD_INV, n, batch, nega, (const cplx<T>*)table, io, st, who);
+}
+
+template <typename T>
+int r2r(void* real, void* temp, const void* table, const gpufft_config* cfg, int batch, int mult,
+        const char* who)
+{
+    if (int s = check_cfg(cfg, GPUFFT_MIN_N_POWER, GPUFFT_MAX_N_POWER, who)) return s;
+    if (int s = check_batch(batch, who)) return s;
+    if (!real || !temp || !table) return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": NULL buffer");
+    const int n = cfg->n_power;
+    const long long L = 1LL << n;
+    const int nega = cfg->reduction_poly == GPUFFT_X_N_PLUS;
+    hipStream_t st = (hipStream_t)cfg->stream;
+    cplx<T>* t = (cplx<T>*)temp;
+    TransformIO<T> io{};
+    io.work = t;
+    if (cfg->fft_type == GPUFFT_FORWARD) {
+        io.lk = LD_REAL;
+        io.src = real;
+        io.src_stride = L;
+        io.sk = ST_CPLX;
+        io.dst = t;
+        io.dst_stride = L;
+        return run_transform<T>(MODE_STD_FWD, n, batch, nega, (const cplx<T>*)table, io, st, who);
+    }
+    io.lk = mult ? LD_MUL : LD_CPLX;
+    io.src = t;
+    io.src2 = t + (long long)batch * L;
+    io.src_stride = L;
+    io.sk = ST_REAL;
+    io.dst = real;
+    io.dst_stride = L;
+    io.do_scale = true;
+    io.scale = mk<T>((T)cfg->mod_inverse_re, (T)cfg->mod_inverse_im);
+    return run_transform<T>(MODE_STD_INV, n, batch, nega, (const cplx<T>*)table, io, st, who);
+}
+
+template <typename T>
+int ffnt(void* inout, void* temp, const void* twist, const void* table, const gpufft_config* cfg,
+         int batch, int mult, const char* who)
+{
+    if (int s = check_cfg(cfg, GPUFFT_MIN_N_POWER_FFNT, GPUFFT_MAX_N_POWER, who)) return s;
+    if (int s = check_batch(batch, who)) return s;
+    if (!inout || !temp || !twist || !table)
+        return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": NULL buffer");
+    const int n = cfg->n_power - 1;  // length of the complex transform
+    const long long L = 1LL << n;
+    hipStream_t st = (hipStream_t)cfg->stream;
+    cplx<T>* t = (cplx<T>*)temp;
+    TransformIO<T> io{};
+    io.work = t;
+    if (cfg->fft_type == GPUFFT_FORWARD) {
+        io.lk = LD_FOLD;
+        io.src = inout;
+        io.ltab = (const cplx<T>*)twist;
+        io.src_stride = 2 * L;
+        io.sk = ST_CPLX;
+        io.dst = t;
+        io.dst_stride = L;
+        return run_transform<T>(MODE_STD_FWD, n, batch, 0, (const cplx<T>*)table, io, st, who);
+    }
+    io.lk = mult ? LD_MUL : LD_CPLX;
+    io.src = t;
+    io.src2 = t + (long long)batch * L;
+    io.src_stride = L;
+    io.sk = ST_UNFOLD;
+    io.dst = inout;
+    io.stab = (const cplx<T>*)twist;
+    io.dst_stride = 2 * L;
+    io.do_scale = true;
+    io.scale = mk<T>((T)cfg->mod_inverse_re, (T)cfg->mod_inverse_im);
+    return run_transform<T>(MODE_STD_INV, n, batch, 0, (const cplx<T>*)table, io, st, who);
+}
+
+template <typename T>
+int special(void* inout, const void* table, const gpufft_config* cfg, int batch, const char* who)
+{
+    if (int s = check_cfg(cfg, GPUFFT_MIN_N_POWER, GPUFFT_MAX_N_POWER, who)) return s;
+    if (int s = check_batch(batch, who)) return s;
+    if (!inout || !table) return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": NULL buffer");
+    const int n = cfg->n_power;
+    const long long L = 1LL << n;
+    hipStream_t st = (hipStream_t)cfg->stream;
+    cplx<T>* x = (cplx<T>*)inout;
+    TransformIO<T> io{};
+    io.lk = LD_CPLX;
+    io.src = x;
+    io.src_stride = L;
+    io.sk = ST_CPLX;
+    io.dst = x;
+    io.dst_stride = L;
+    io.work = x;
+    if (cfg->fft_type == GPUFFT_FORWARD)
+        return run_transform<T>(MODE_SPE_FWD, n, batch, 0, (const cplx<T>*)table, io, st, who);
+    io.do_scale = true;
+    io.scale = mk<T>((T)cfg->mod_inverse_re, (T)cfg->mod_inverse_im);
+    return run_transform<T>(MODE_SPE_INV, n, batch, 0, (const cplx<T>*)table, io, st, who);
+}
+
+}  // namespace
+
+extern "C" {
+
+int gpufft_fft_c2c_f32(void* inout, const void* t, const gpufft_config* cfg, int batch, int mult)
+{
+    return c2c<float>(inout, t, cfg, batch, mult, "gpufft_fft_c2c_f32");
+}
+int gpufft_fft_c2c_f64(void* inout, const void* t, const gpufft_config* cfg, int batch, int mult)
+{
+    return c2c<double>(inout, t, cfg, batch, mult, "gpufft_fft_c2c_f64");
+}
+int gpufft_fft_r2r_f32(void* real, void* temp, const void* t, const gpufft_config* cfg, int batch,
+                       int mult)
+{
+    return r2r<float>(real, temp, t, cfg, batch, mult, "gpufft_fft_r2r_f32");
+}
+int gpufft_fft_r2r_f64(void* real, void* temp, const void* t, const gpufft_config* cfg, int batch,
+                       int mult)
+{
+    return r2r<double>(real, temp, t, cfg, batch, mult, "gpufft_fft_r2r_f64");
+}
+int gpufft_ffnt_f32(void* inout, void* temp, const void* tw, const void* t, const gpufft_config* cfg,
+                    int batch, int mult)
+{
+    return ffnt<float>(inout, temp, tw, t, cfg, batch, mult, "gpufft_ffnt_f32");
+}
+int gpufft_ffnt_f64(void* inout, void* temp, const void* tw, const void* t, const gpufft_config* cfg,
+                    int batch, int mult)
+{
+    return ffnt<double>(inout, temp, tw, t, cfg, batch, mult, "gpufft_ffnt_f64");
+}
+int gpufft_special_fft_f32(void* inout, const void* t, const gpufft_config* cfg, int batch)
+{
+    return special<float>(inout, t, cfg, batch, "gpufft_special_fft_f32");
+}
+int gpufft_special_fft_f64(void* inout, const void* t, const gpufft_config* cfg, int batch)
+{
+    return special<double>(inout, t, cfg, batch, "gpufft_special_fft_f64");
+}
+
+}  // extern "C"

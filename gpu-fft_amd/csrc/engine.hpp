@@ -1,0 +1,270 @@
+// Generic LDS-tiled radix pass for gfx950.
+//
+// A transform of length L = 2^n is a sequence of radix-2 stages, each one
+// pairing positions p and p ^ 2^b for one position bit b:
+//   MODE_STD_FWD  Cooley-Tukey, bits n-1 .. 0, twiddle tab[base + (p >> (b+1))]
+//                 (ForwardCore, reference src/lib/fft.cu:11-152; base = 2^(n-1-b)
+//                 for X^N+1, 0 for X^N-1, fft.cu:67-74)
+//   MODE_STD_INV  Gentleman-Sande, bits 0 .. n-1, same twiddle index
+//                 (InverseCore, fft.cu:154-260)
+//   MODE_SPE_FWD  Cooley-Tukey, bits 0 .. n-1, twiddle tab[2^b + (p & (2^b-1))]
+//                 (Special_ForwardCore, fft.cu:856-926)
+//   MODE_SPE_INV  Gentleman-Sande, bits n-1 .. 0, same twiddle index
+//                 (Special_InverseCore, fft.cu:727-854)
+//
+// One launch ("pass") applies the stages of K consecutive bits [blo, blo+K).
+// A workgroup owns a tile of 2^LOGT elements: 2^K "rows" (the pass bits) by
+// 2^logC "columns" (consecutive positions below blo, so that every global
+// row segment is a contiguous run), or, when the pass covers the whole
+// transform, 2^logBT whole transforms.  The tile is read from HBM once with
+// consecutive lanes on consecutive addresses, kept in LDS (XOR-swizzled in
+// 16-element blocks against bank conflicts) while each thread performs
+// radix-2^E steps in registers, and written back once.  The first pass can
+// fuse the operation's input transform (real promotion, pointwise product,
+// FFNT fold+twist) and the last pass its output transform (scale, real part,
+// FFNT untwist+unfold), so a single-pass transform touches HBM once each way.
+#pragma once
+
+#include "cplx.hpp"
+
+namespace gfa {
+
+enum Mode { MODE_STD_FWD = 0, MODE_STD_INV = 1, MODE_SPE_FWD = 2, MODE_SPE_INV = 3 };
+enum LoadKind { LD_CPLX = 0, LD_REAL = 1, LD_MUL = 2, LD_FOLD = 3 };
+enum StoreKind { ST_CPLX = 0, ST_REAL = 1, ST_UNFOLD = 2 };
+
+__host__ __device__ constexpr bool mode_is_ct(int m) { return m == MODE_STD_FWD || m == MODE_SPE_FWD; }
+__host__ __device__ constexpr bool mode_descending(int m) { return m == MODE_STD_FWD || m == MODE_SPE_INV; }
+__host__ __device__ constexpr bool mode_special(int m) { return m == MODE_SPE_FWD || m == MODE_SPE_INV; }
+
+template <typename T> struct PassArgs {
+    int n;       // log2 transform length
+    int blo;     // lowest bit of this pass
+    int K;       // number of bits (stages) of this pass
+    int logC;    // log2 columns per tile (0 when the pass reaches bit 0)
+    int logBT;   // log2 transforms per tile (only when K == n)
+    int batch;   // transforms in the launch
+    int nega;    // X^N+1 twiddle base (standard modes only)
+    int do_scale;
+    cplx<T> scale;
+    const cplx<T>* tab;  // root-of-unity table (reference layout)
+    // input
+    const void* src;
+    const void* src2;      // second operand (LD_MUL)
+    const cplx<T>* ltab;   // twist table (LD_FOLD)
+    long long src_stride;  // elements between transforms
+    // output
+    void* dst;
+    const cplx<T>* stab;  // untwist table (ST_UNFOLD)
+    long long dst_stride;
+};
+
+// Bank-conflict swizzle: permutes each aligned block of 16 elements by the
+// XOR of the next two 4-bit fields of the index.
+__device__ __forceinline__ int swz(int i) { return i ^ (((i >> 4) ^ (i >> 8)) & 15); }
+
+template <typename T, int LK>
+__device__ __forceinline__ cplx<T> load_elem(const PassArgs<T>& a, long long b, int p)
+{
+    if constexpr (LK == LD_CPLX) {
+        return ldc(reinterpret_cast<const cplx<T>*>(a.src) + b * a.src_stride + p);
+    } else if constexpr (LK == LD_REAL) {
+        return mk<T>(reinterpret_cast<const T*>(a.src)[b * a.src_stride + p], T(0));
+    } else if constexpr (LK == LD_MUL) {
+        cplx<T> u = ldc(reinterpret_cast<const cplx<T>*>(a.src) + b * a.src_stride + p);
+        cplx<T> v = ldc(reinterpret_cast<const cplx<T>*>(a.src2) + b * a.src_stride + p);
+        return u * v;
+    } else {  // LD_FOLD: z = (f[p] + i f[p+L]) * twist[p]
+        const T* f = reinterpret_cast<const T*>(a.src) + b * a.src_stride;
+        int L = 1 << a.n;
+        return mk<T>(f[p], f[p + L]) * ldc(a.ltab + p);
+    }
+}
+
+template <typename T, int SK>
+__device__ __forceinline__ void store_elem(const PassArgs<T>& a, long long b, int p, cplx<T> z)
+{
+    if (a.do_scale) z = z * a.scale;
+    if constexpr (SK == ST_CPLX) {
+        stc(reinterpret_cast<cplx<T>*>(a.dst) + b * a.dst_stride + p, z);
+    } else if constexpr (SK == ST_REAL) {
+        reinterpret_cast<T*>(a.dst)[b * a.dst_stride + p] = z.x;
+    } else {  // ST_UNFOLD: (z * untwist[p]) -> out[p] = re, out[p+L] = im
+        z = z * ldc(a.stab + p);
+        T* o = reinterpret_cast<T*>(a.dst) + b * a.dst_stride;
+        int L = 1 << a.n;
+        o[p] = z.x;
+        o[p + L] = z.y;
+    }
+}
+
+// Position (within its transform) of tile row r, column col.
+struct TileGeo {
+    int hi_pos;   // (hi << (blo+K)) | c0
+    int blo, K, logC;
+    __device__ __forceinline__ int pos(int r, int col) const
+    {
+        return hi_pos | ((r & ((1 << K) - 1)) << blo) | col;
+    }
+};
+
+template <typename T, int MODE>
+__device__ __forceinline__ int twiddle_index(int n, int nega, int b, int p)
+{
+    if constexpr (mode_special(MODE)) {
+        return (1 << b) + (p & ((1 << b) - 1));
+    } else {
+        int base = nega ? (1 << (n - 1 - b)) : 0;
+        return base + (p >> (b + 1));
+    }
+}
+
+// One radix-2^RHO step over tile-row bits [q, q+RHO): every thread takes
+// EPT/2^RHO groups of 2^RHO elements from LDS, runs the RHO stages in
+// registers and writes them back.
+template <typename T, int MODE, int E, int RHO, int NT>
+__device__ __forceinline__ void radix_step(cplx<T>* lds, const PassArgs<T>& a, const TileGeo& g,
+                                           int q, int tid)
+{
+    constexpr int EPT = 1 << E;
+    constexpr int GS = 1 << RHO;       // group size
+    constexpr int GPT = EPT / GS;      // groups per thread
+    const int logC = g.logC;
+    const int C = 1 << logC;
+#pragma unroll
+    for (int k = 0; k < GPT; k++) {
+        int gamma = tid + k * NT;
+        int col = gamma & (C - 1);
+        int rest = gamma >> logC;
+        int rlo = rest & ((1 << q) - 1);
+        int rhi = rest >> q;
+        int rbase = (rhi << (q + RHO)) | rlo;
+        cplx<T> v[GS];
+        int li[GS];
+#pragma unroll
+        for (int j = 0; j < GS; j++) {
+            int r = rbase | (j << q);
+            li[j] = swz((r << logC) | col);
+            v[j] = lds[li[j]];
+        }
+#pragma unroll
+        for (int st = 0; st < RHO; st++) {
+            // bit inside the group handled by this stage
+            const int i = mode_descending(MODE) ? (RHO - 1 - st) : st;
+            const int bpos = g.blo + q + i;  // position bit
+            // Standard twiddles depend only on the group bits above i, special
+            // ones only on the bits below i: load each distinct one once.
+#pragma unroll
+            for (int outer = 0; outer < (mode_special(MODE) ? (1 << i) : (GS >> (i + 1))); outer++) {
+                const int j_first = mode_special(MODE) ? outer : (outer << (i + 1));
+                int p0 = g.pos(rbase | (j_first << q), col);
+                cplx<T> w = ldc(a.tab + twiddle_index<T, MODE>(a.n, a.nega, bpos, p0));
+#pragma unroll
+                for (int inner = 0; inner < (mode_special(MODE) ? (GS >> (i + 1)) : (1 << i)); inner++) {
+                    const int j0 = mode_special(MODE) ? ((inner << (i + 1)) | outer)
+                                                      : ((outer << (i + 1)) | inner);
+                    const int j1 = j0 | (1 << i);
+                    cplx<T> U = v[j0], V = v[j1];
+                    if constexpr (mode_is_ct(MODE)) {
+                        cplx<T> Vw = V * w;
+                        v[j0] = U + Vw;
+                        v[j1] = U - Vw;
+                    } else {
+                        v[j0] = U + V;
+                        v[j1] = (U - V) * w;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < GS; j++) lds[li[j]] = v[j];
+    }
+}
+
+template <typename T, int MODE, int E, int NT>
+__device__ __forceinline__ void run_steps(cplx<T>* lds, const PassArgs<T>& a, const TileGeo& g,
+                                          int tid)
+{
+    // K stages as steps of E bits plus one remainder step; descending modes
+    // start from the top tile-row bits, ascending modes from bit 0.
+    const int K = a.K;
+    const int full = K / E;
+    const int rem = K - full * E;
+    for (int s = 0; s < full + (rem ? 1 : 0); s++) {
+        int rho = (s < full) ? E : rem;
+        int q;
+        if (mode_descending(MODE)) {
+            q = K - s * E - rho;  // steps cover [K-E, K), [K-2E, K-E), ..., [0, rem)
+        } else {
+            q = s * E;            // steps cover [0, E), [E, 2E), ..., [full*E, K)
+        }
+        __syncthreads();
+        switch (rho) {
+            case 1: radix_step<T, MODE, E, 1, NT>(lds, a, g, q, tid); break;
+            case 2: if constexpr (E >= 2) radix_step<T, MODE, E, 2, NT>(lds, a, g, q, tid); break;
+            case 3: if constexpr (E >= 3) radix_step<T, MODE, E, 3, NT>(lds, a, g, q, tid); break;
+            case 4: if constexpr (E >= 4) radix_step<T, MODE, E, 4, NT>(lds, a, g, q, tid); break;
+            default: break;
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int MODE, int LOGT, int E, int LK, int SK>
+__global__ __launch_bounds__((1 << LOGT) >> E) void pass_kernel(PassArgs<T> a)
+{
+    constexpr int TILE = 1 << LOGT;
+    constexpr int EPT = 1 << E;
+    constexpr int NT = TILE / EPT;
+    __shared__ cplx<T> lds[TILE];
+
+    const int tid = threadIdx.x;
+    const int n = a.n;
+    const int logC = a.logC;
+    const int bhi = a.blo + a.K;
+    TileGeo g;
+    g.blo = a.blo;
+    g.K = a.K;
+    g.logC = logC;
+
+    // tile -> (transform block, hi bits above the pass, column block)
+    const int ncb_log = a.blo - logC;
+    const int nhi_log = n - bhi;
+    long long tile = blockIdx.x;
+    const int cb = (int)(tile & ((1LL << ncb_log) - 1));
+    tile >>= ncb_log;
+    const int hi = (int)(tile & ((1LL << nhi_log) - 1));
+    tile >>= nhi_log;
+    const long long bt0 = tile << a.logBT;  // first transform of this tile
+    g.hi_pos = (hi << bhi) | (cb << logC);
+    const int C = 1 << logC;
+    const int Kmask = (1 << a.K) - 1;
+
+    // ---- load: consecutive lanes on consecutive positions
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+        int idx = tid + i * NT;
+        int col = idx & (C - 1);
+        int rext = idx >> logC;
+        long long b = bt0 + (rext >> a.K);
+        int p = g.pos(rext & Kmask, col);
+        cplx<T> v = mk<T>(T(0), T(0));
+        if (b < a.batch) v = load_elem<T, LK>(a, b, p);
+        lds[swz(idx)] = v;
+    }
+
+    run_steps<T, MODE, E, NT>(lds, a, g, tid);
+
+    // ---- store
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+        int idx = tid + i * NT;
+        int col = idx & (C - 1);
+        int rext = idx >> logC;
+        long long b = bt0 + (rext >> a.K);
+        int p = g.pos(rext & Kmask, col);
+        if (b < a.batch) store_elem<T, SK>(a, b, p, lds[swz(idx)]);
+    }
+}
+
+}  // namespace gfa

@@ -1,0 +1,161 @@
+// Host-side planning and launch of a full transform as a chain of passes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "engine.hpp"
+#include "status.hpp"
+
+namespace gfa {
+
+// Tile geometry per element type: 2^LOGT elements of 16 bytes (double) or
+// 8 bytes (float) = 64 KiB of LDS, 16 elements per thread, 256 / 512 threads.
+template <typename T> struct TileCfg;
+template <> struct TileCfg<double> {
+    static constexpr int LOGT = 12;
+    static constexpr int E = 4;
+};
+template <> struct TileCfg<float> {
+    static constexpr int LOGT = 13;
+    static constexpr int E = 4;
+};
+
+struct PassPlan {
+    int blo, K, logC, logBT;
+};
+
+// Splits the n position bits into passes of at most LOGT bits.  The pass
+// that reaches bit 0 takes LOGT bits (contiguous tiles); the others take at
+// most LOGT-4 bits so that every tile row is at least 16 contiguous elements.
+// Descending modes run the top bits first, ascending modes bit 0 first.
+inline std::vector<PassPlan> plan_passes(int n, int LOGT, bool descending)
+{
+    std::vector<PassPlan> p;
+    if (n <= LOGT) {
+        p.push_back({0, n, 0, LOGT - n});
+        return p;
+    }
+    int rest = n - LOGT;
+    int maxk = LOGT - 4;
+    int cnt = (rest + maxk - 1) / maxk;
+    std::vector<int> ks;
+    for (int i = 0; i < cnt; i++) ks.push_back(rest / cnt + (i < rest % cnt ? 1 : 0));
+    // ascending order of bits: [0,LOGT), then the chunks upward
+    std::vector<PassPlan> asc;
+    asc.push_back({0, LOGT, 0, 0});
+    int b = LOGT;
+    for (int k : ks) {
+        asc.push_back({b, k, LOGT - k, 0});
+        b += k;
+    }
+    if (descending) {
+        for (int i = (int)asc.size() - 1; i >= 0; i--) p.push_back(asc[i]);
+    } else {
+        p = asc;
+    }
+    return p;
+}
+
+// What the first pass reads and the last pass writes.
+template <typename T> struct TransformIO {
+    int lk;                 // LoadKind of the first pass
+    const void* src;
+    const void* src2;
+    const cplx<T>* ltab;
+    long long src_stride;
+    int sk;                 // StoreKind of the last pass
+    void* dst;
+    const cplx<T>* stab;
+    long long dst_stride;
+    cplx<T>* work;          // complex [batch][2^n] buffer for intermediate passes
+    bool do_scale;
+    cplx<T> scale;
+};
+
+template <typename T>
+using PassFn = void (*)(PassArgs<T>);
+
+template <typename T, int MODE, int LK, int SK> inline PassFn<T> pass_fn()
+{
+    return &pass_kernel<T, MODE, TileCfg<T>::LOGT, TileCfg<T>::E, LK, SK>;
+}
+
+// Kernel for (mode, load kind, store kind); nullptr for combinations no
+// operation uses.
+template <typename T> PassFn<T> select_pass(int mode, int lk, int sk)
+{
+#define GFA_SEL(M, L, S) \
+    if (mode == M && lk == L && sk == S) return pass_fn<T, M, L, S>();
+    GFA_SEL(MODE_STD_FWD, LD_CPLX, ST_CPLX)
+    GFA_SEL(MODE_STD_FWD, LD_REAL, ST_CPLX)
+    GFA_SEL(MODE_STD_FWD, LD_FOLD, ST_CPLX)
+    GFA_SEL(MODE_STD_INV, LD_CPLX, ST_CPLX)
+    GFA_SEL(MODE_STD_INV, LD_MUL, ST_CPLX)
+    GFA_SEL(MODE_STD_INV, LD_CPLX, ST_REAL)
+    GFA_SEL(MODE_STD_INV, LD_MUL, ST_REAL)
+    GFA_SEL(MODE_STD_INV, LD_CPLX, ST_UNFOLD)
+    GFA_SEL(MODE_STD_INV, LD_MUL, ST_UNFOLD)
+    GFA_SEL(MODE_SPE_FWD, LD_CPLX, ST_CPLX)
+    GFA_SEL(MODE_SPE_INV, LD_CPLX, ST_CPLX)
+#undef GFA_SEL
+    return nullptr;
+}
+
+// Launches every pass of one transform of length 2^n over `batch` transforms.
+template <typename T>
+int run_transform(int mode, int n, int batch, int nega, const cplx<T>* tab,
+                  const TransformIO<T>& io, hipStream_t stream, const char* who)
+{
+    constexpr int LOGT = TileCfg<T>::LOGT;
+    constexpr int NT = (1 << LOGT) >> TileCfg<T>::E;
+    std::vector<PassPlan> plan = plan_passes(n, LOGT, mode_descending(mode));
+    const long long L = 1LL << n;
+    for (size_t i = 0; i < plan.size(); i++) {
+        const bool first = (i == 0), last = (i + 1 == plan.size());
+        const PassPlan& pp = plan[i];
+        PassArgs<T> a{};
+        a.n = n;
+        a.blo = pp.blo;
+        a.K = pp.K;
+        a.logC = pp.logC;
+        a.logBT = pp.logBT;
+        a.batch = batch;
+        a.nega = nega;
+        a.tab = tab;
+        int lk = LD_CPLX, sk = ST_CPLX;
+        if (first) {
+            lk = io.lk;
+            a.src = io.src;
+            a.src2 = io.src2;
+            a.ltab = io.ltab;
+            a.src_stride = io.src_stride;
+        } else {
+            a.src = io.work;
+            a.src_stride = L;
+        }
+        if (last) {
+            sk = io.sk;
+            a.dst = io.dst;
+            a.stab = io.stab;
+            a.dst_stride = io.dst_stride;
+            a.do_scale = io.do_scale ? 1 : 0;
+            a.scale = io.scale;
+        } else {
+            a.dst = io.work;
+            a.dst_stride = L;
+        }
+        PassFn<T> fn = select_pass<T>(mode, lk, sk);
+        if (!fn) return fail(GPUFFT_ERR_UNSUPPORTED, std::string(who) + ": no kernel for this pass");
+        long long tiles = ((long long)batch << n) >> LOGT;
+        if (pp.logBT > 0) tiles = ((long long)batch + (1LL << pp.logBT) - 1) >> pp.logBT;
+        if (tiles <= 0) tiles = 1;
+        if (tiles > 0x7fffffffLL) return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": batch too large");
+        hipLaunchKernelGGL(fn, dim3((unsigned)tiles), dim3(NT), 0, stream, a);
+        GFA_CHECK_LAUNCH(who);
+    }
+    return ok();
+}
+
+}  // namespace gfa

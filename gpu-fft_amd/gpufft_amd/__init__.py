@@ -1,0 +1,172 @@
+"""Python binding of libgpufft_amd.so (the MI355X GPU-FFT C ABI) via ctypes.
+
+This is the host-side mirror of the reference's GPU entry points
+(Alisah-Ozcan/GPU-FFT, src/include/gpufft/fft.cuh): the same operations,
+argument meaning and errors (`ValueError` where the reference throws
+std::invalid_argument, `HipError` where it throws gpufft::CudaException).
+Device memory comes from torch tensors (torch is only the allocator and the
+stream provider here); every call goes through the compiled C ABI.
+
+There is no CPU fallback: if the shared library is missing, importing the
+binding raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libgpufft_amd.so")
+
+FORWARD, INVERSE = 0, 1
+X_N_PLUS, X_N_MINUS = 0, 1
+
+OK = 0
+ERR_INVALID_ARGUMENT = 1
+ERR_HIP = 2
+ERR_NULL_POINTER = 3
+ERR_UNSUPPORTED = 4
+
+MIN_N_POWER = 1
+MIN_N_POWER_FFNT = 2
+MAX_N_POWER = 24
+
+
+class HipError(RuntimeError):
+    """A HIP runtime error reported by the library (gpufft::CudaException)."""
+
+
+class Config(ctypes.Structure):
+    """gpufft_config (include/gpufft_amd.h) = fft_configuration<T>."""
+    _fields_ = [
+        ("n_power", ctypes.c_int),
+        ("fft_type", ctypes.c_int),
+        ("reduction_poly", ctypes.c_int),
+        ("zero_padding", ctypes.c_int),
+        ("mod_inverse_re", ctypes.c_double),
+        ("mod_inverse_im", ctypes.c_double),
+        ("stream", ctypes.c_void_p),
+    ]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            "libgpufft_amd.so not built: run `make -C gpu-fft_amd` "
+            "(or __graft_entry__.build()) first; there is no fallback path")
+    return ctypes.CDLL(LIB_PATH)
+
+
+_lib = _load()
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_CP = ctypes.POINTER(Config)
+
+_SIGS = {
+    "gpufft_fft_c2c_f32": [_P, _P, _CP, _I, _I],
+    "gpufft_fft_c2c_f64": [_P, _P, _CP, _I, _I],
+    "gpufft_fft_r2r_f32": [_P, _P, _P, _CP, _I, _I],
+    "gpufft_fft_r2r_f64": [_P, _P, _P, _CP, _I, _I],
+    "gpufft_ffnt_f32": [_P, _P, _P, _P, _CP, _I, _I],
+    "gpufft_ffnt_f64": [_P, _P, _P, _P, _CP, _I, _I],
+    "gpufft_special_fft_f32": [_P, _P, _CP, _I],
+    "gpufft_special_fft_f64": [_P, _P, _CP, _I],
+}
+
+
+def _declare():
+    for name, args in _SIGS.items():
+        f = getattr(_lib, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    _lib.gpufft_last_error_message.argtypes = []
+    _lib.gpufft_last_error_message.restype = ctypes.c_char_p
+    _lib.gpufft_last_hip_error.argtypes = []
+    _lib.gpufft_last_hip_error.restype = ctypes.c_int
+
+
+_declare()
+
+
+def lib():
+    return _lib
+
+
+def exported_symbols():
+    return sorted(_SIGS)
+
+
+def check(status):
+    """Raise the reference's exception type for a non-zero status."""
+    if status == OK:
+        return
+    msg = _lib.gpufft_last_error_message().decode()
+    if status == ERR_HIP:
+        raise HipError(msg)
+    raise ValueError(msg)
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return t
+    return t.data_ptr()
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+def _prec(t):
+    """Element precision of a tensor argument; raw pointers mean double."""
+    if t is None or isinstance(t, int):
+        return "f64"
+    import torch
+    if t.dtype in (torch.complex64, torch.float32, torch.int32):
+        return "f32"
+    return "f64"
+
+
+def config(n_power, fft_type, reduction_poly=X_N_MINUS, mod_inverse=1.0, stream=None,
+           zero_padding=False):
+    c = Config()
+    c.n_power = n_power
+    c.fft_type = fft_type
+    c.reduction_poly = reduction_poly
+    c.zero_padding = 1 if zero_padding else 0
+    mi = complex(mod_inverse)
+    c.mod_inverse_re = mi.real
+    c.mod_inverse_im = mi.imag
+    c.stream = _stream(stream)
+    return c
+
+
+# ---- drop-in operations (names of the reference's GPU entry points) ----
+def GPU_FFT(inout, root_table, cfg, batch_size, multiplication=False):
+    """Complex-to-complex GPU_FFT (reference fft.cuh:121-124)."""
+    fn = getattr(_lib, "gpufft_fft_c2c_" + _prec(inout))
+    check(fn(_ptr(inout), _ptr(root_table), ctypes.byref(cfg), batch_size, int(multiplication)))
+
+
+def GPU_FFT_R(real_inout, temp, root_table, cfg, batch_size, multiplication=False):
+    """Real-to-complex / complex-to-real GPU_FFT (reference fft.cuh:126-132)."""
+    fn = getattr(_lib, "gpufft_fft_r2r_" + _prec(temp))
+    check(fn(_ptr(real_inout), _ptr(temp), _ptr(root_table), ctypes.byref(cfg), batch_size,
+             int(multiplication)))
+
+
+def GPU_FFNT(inout, temp, twist_table, root_table, cfg, batch_size, multiplication=False):
+    """GPU_FFNT (reference fft.cuh:269-273); pass the untwist table for INVERSE."""
+    fn = getattr(_lib, "gpufft_ffnt_" + _prec(temp))
+    check(fn(_ptr(inout), _ptr(temp), _ptr(twist_table), _ptr(root_table), ctypes.byref(cfg),
+             batch_size, int(multiplication)))
+
+
+def GPU_Special_FFT(inout, root_table, cfg, batch_size):
+    """GPU_Special_FFT (reference fft.cuh:194-197)."""
+    fn = getattr(_lib, "gpufft_special_fft_" + _prec(inout))
+    check(fn(_ptr(inout), _ptr(root_table), ctypes.byref(cfg), batch_size))

@@ -1,0 +1,115 @@
+/*
+ * gpufft_amd — C ABI of the MI355X (gfx950) GPU-FFT polynomial-multiplication
+ * library.
+ *
+ * Plain pointers, sizes and a POD configuration; no C++ types, no exceptions.
+ * Every entry point returns a status (GPUFFT_OK on success).  All work is
+ * enqueued asynchronously on cfg->stream (NULL = the default stream); nothing
+ * synchronises.  Device buffers and root tables are caller-owned device
+ * memory in the layouts of the reference library (Alisah-Ozcan/GPU-FFT):
+ * complex values are interleaved (re, im) pairs of float (8 bytes) or
+ * double (16 bytes), as COMPLEX<T> in src/include/gpufft/complex.cuh:14-16.
+ *
+ * The C++ templates of include/gpufft/fft.cuh (the reference's public
+ * interface) are inline wrappers over these functions.
+ */
+#ifndef GPUFFT_AMD_H
+#define GPUFFT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define GPUFFT_OK 0
+#define GPUFFT_ERR_INVALID_ARGUMENT 1 /* maps to std::invalid_argument */
+#define GPUFFT_ERR_HIP 2              /* a HIP runtime error; see gpufft_last_hip_error() */
+#define GPUFFT_ERR_NULL_POINTER 3     /* maps to std::invalid_argument */
+#define GPUFFT_ERR_UNSUPPORTED 4      /* maps to std::invalid_argument */
+
+/* ---- configuration ----
+ * Mirrors fft_configuration<T> (reference src/include/gpufft/fft.cuh:43-51)
+ * with the enum values of fft.cuh:37-41 and fft_cpu.cuh:16-20. */
+#define GPUFFT_FORWARD 0
+#define GPUFFT_INVERSE 1
+#define GPUFFT_X_N_PLUS 0  /* X^N + 1 (negacyclic) */
+#define GPUFFT_X_N_MINUS 1 /* X^N - 1 (cyclic) */
+
+typedef struct gpufft_config {
+    int n_power;        /* log2 transform length (FFNT: log2 ring size) */
+    int fft_type;       /* GPUFFT_FORWARD / GPUFFT_INVERSE */
+    int reduction_poly; /* GPUFFT_X_N_PLUS / GPUFFT_X_N_MINUS */
+    int zero_padding;   /* accepted and ignored, as in the reference */
+    double mod_inverse_re, mod_inverse_im; /* complex scale of INVERSE */
+    void* stream;       /* hipStream_t */
+} gpufft_config;
+
+/* ---- supported ranges of n_power (the reference accepts 12..24 for
+ * GPU_FFT/GPU_FFNT and 11..24 for GPU_Special_FFT; this library extends the
+ * lower end and keeps the upper one) ---- */
+#define GPUFFT_MIN_N_POWER 1
+#define GPUFFT_MIN_N_POWER_FFNT 2
+#define GPUFFT_MAX_N_POWER 24
+
+/* ---- drop-in transforms ---- */
+
+/* Complex-to-complex transform, in place on inout[batch][2^n_power].
+ * FORWARD: Cooley-Tukey, natural order in, bit-reversed order out, twiddle
+ *   table in the FFT<T>::ReverseRootTable() layout (X^N-1: 2^(n_power-1)
+ *   entries; X^N+1: 2^n_power entries).
+ * INVERSE: Gentleman-Sande, bit-reversed in, natural out, times mod_inverse.
+ *   With multiplication != 0 the input is inout[b] * inout[b + batch] and
+ *   the result goes to inout[b], b < batch.
+ * Replaces gpufft::GPU_FFT(COMPLEX<T>*, COMPLEX<T>*, fft_configuration<T>,
+ * int, bool), reference src/include/gpufft/fft.cuh:121-124, src/lib/fft.cu:517-598. */
+int gpufft_fft_c2c_f32(void* inout, const void* root_table, const gpufft_config* cfg,
+                       int batch, int multiplication);
+int gpufft_fft_c2c_f64(void* inout, const void* root_table, const gpufft_config* cfg,
+                       int batch, int multiplication);
+
+/* Real transform through a complex buffer.
+ * FORWARD: real[batch][2^n] in, full bit-reversed complex spectrum out in
+ *   temp[batch][2^n]; real is not modified.
+ * INVERSE: temp[batch][2^n] (times temp[b + batch] with multiplication) in,
+ *   real[b][i] = Re(x[i] * mod_inverse) out; temp is used as scratch.
+ * Replaces gpufft::GPU_FFT(T*, COMPLEX<T>*, COMPLEX<T>*, fft_configuration<T>,
+ * int, bool), reference fft.cuh:126-132, fft.cu:600-725. */
+int gpufft_fft_r2r_f32(void* real_inout, void* temp, const void* root_table,
+                       const gpufft_config* cfg, int batch, int multiplication);
+int gpufft_fft_r2r_f64(void* real_inout, void* temp, const void* root_table,
+                       const gpufft_config* cfg, int batch, int multiplication);
+
+/* FFNT negacyclic transform (ring size N = 2^n_power, length-N/2 complex FFT).
+ * FORWARD: real inout[batch][N] -> fold z_j = f_j + i f_{j+N/2}, times
+ *   twist_table[j], forward FFT -> temp[batch][N/2] (bit-reversed).
+ * INVERSE: temp (times temp[b + batch] with multiplication) -> inverse FFT,
+ *   times mod_inverse, times twist_table[j] (pass the untwist table here),
+ *   unfold -> inout[b][j] = Re, inout[b][j + N/2] = Im (not rounded).
+ * Replaces gpufft::GPU_FFNT, reference fft.cuh:269-273, fft.cu:1386-1515. */
+int gpufft_ffnt_f32(void* inout, void* temp, const void* twist_table, const void* root_table,
+                    const gpufft_config* cfg, int batch, int multiplication);
+int gpufft_ffnt_f64(void* inout, void* temp, const void* twist_table, const void* root_table,
+                    const gpufft_config* cfg, int batch, int multiplication);
+
+/* Special (CKKS canonical-embedding) FFT, in place on inout[batch][2^n_power].
+ * FORWARD: bit-reversed in, natural out, w_j = sum_k z_k zeta^{(5^j mod 4l) k}.
+ * INVERSE: natural in, bit-reversed out, times mod_inverse.
+ * Table layout: T[h + j] = exp(+-2 pi i (5^j mod 8h) / 8h), h = 1..l/2, j < h.
+ * Replaces gpufft::GPU_Special_FFT, reference fft.cuh:194-197, fft.cu:928-1007. */
+int gpufft_special_fft_f32(void* inout, const void* root_table, const gpufft_config* cfg,
+                           int batch);
+int gpufft_special_fft_f64(void* inout, const void* root_table, const gpufft_config* cfg,
+                           int batch);
+
+/* ---- errors ---- */
+/* Human-readable description of the last failure on the calling thread. */
+const char* gpufft_last_error_message(void);
+/* hipError_t value of the last GPUFFT_ERR_HIP on the calling thread. */
+int gpufft_last_hip_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,63 @@
+"""CPU checks of the C ABI boundary: the library loads, exports every symbol
+include/gpufft_amd.h declares, and validates arguments on the host before any
+HIP call (so these run without a GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "gpufft_amd.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gpufft_\w+)\s*\(", text)))
+
+
+def test_header_declares_functions():
+    names = declared_functions()
+    assert "gpufft_fft_c2c_f64" in names and "gpufft_ffnt_f32" in names
+
+
+def test_library_exports_every_declared_symbol():
+    import gpufft_amd as G
+    lib = ctypes.CDLL(G.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def _cfg(G, n_power, fft_type=0):
+    return G.config(n_power, fft_type, stream=0)
+
+
+@pytest.mark.parametrize("n_power", [0, -3, 25, 40])
+def test_invalid_n_power_raises_like_reference(n_power):
+    """fft.cu:522-525 throws std::invalid_argument("Invalid n_power range!")."""
+    import gpufft_amd as G
+    with pytest.raises(ValueError, match="Invalid n_power range!"):
+        G.GPU_FFT(0x1000, 0x2000, _cfg(G, n_power), 1)
+    with pytest.raises(ValueError, match="Invalid n_power range!"):
+        G.GPU_Special_FFT(0x1000, 0x2000, _cfg(G, n_power), 1)
+
+
+def test_ffnt_lower_limit():
+    import gpufft_amd as G
+    with pytest.raises(ValueError, match="Invalid n_power range!"):
+        G.check(G.lib().gpufft_ffnt_f64(0x1000, 0x2000, 0x3000, 0x4000,
+                                         ctypes.byref(_cfg(G, 1)), 1, 0))
+
+
+def test_null_and_batch_validation():
+    import gpufft_amd as G
+    with pytest.raises(ValueError, match="NULL"):
+        G.GPU_FFT(None, 0x2000, _cfg(G, 12), 1)
+    with pytest.raises(ValueError, match="batch"):
+        G.GPU_FFT(0x1000, 0x2000, _cfg(G, 12), 0)
+    with pytest.raises(ValueError, match="fft_type"):
+        G.GPU_FFT(0x1000, 0x2000, _cfg(G, 12, 7), 1)
+    st = G.lib().gpufft_fft_c2c_f64(0x1000, 0x2000, None, 1, 0)
+    assert st == G.ERR_NULL_POINTER
