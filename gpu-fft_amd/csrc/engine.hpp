@@ -102,6 +102,7 @@ __device__ __forceinline__ void store_elem(const PassArgs<T>& a, long long b, in
 struct TileGeo {
     int hi_pos;   // (hi << (blo+K)) | c0
     int blo, K, logC;
+    rsrc_t tab;   // root-of-unity table
     __device__ __forceinline__ int pos(int r, int col) const
     {
         return hi_pos | ((r & ((1 << K) - 1)) << blo) | col;
@@ -119,6 +120,59 @@ __device__ __forceinline__ int twiddle_index(int n, int nega, int b, int p)
     }
 }
 
+// Runs the RHO stages of one group held in registers: v[j] sits at tile row
+// rbase | (j << q), column col.  Standard twiddles depend only on the group
+// bits above the stage bit i, special ones only on the bits below it, so
+// each distinct twiddle is loaded once.
+template <typename T, int MODE, int RHO>
+__device__ __forceinline__ void group_stages(cplx<T>* v, const PassArgs<T>& a, const TileGeo& g,
+                                             int rbase, int col, int q)
+{
+    constexpr int GS = 1 << RHO;
+#pragma unroll
+    for (int st = 0; st < RHO; st++) {
+        const int i = mode_descending(MODE) ? (RHO - 1 - st) : st;
+        const int bpos = g.blo + q + i;  // position bit of this stage
+        // keep the twiddle loads of a stage inside that stage (bounds the
+        // number of registers held by loads in flight)
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int outer = 0; outer < (mode_special(MODE) ? (1 << i) : (GS >> (i + 1))); outer++) {
+            const int j_first = mode_special(MODE) ? outer : (outer << (i + 1));
+            const int p0 = g.pos(rbase | (j_first << q), col);
+            const cplx<T> w = bldc<T>(g.tab, twiddle_index<T, MODE>(a.n, a.nega, bpos, p0));
+#pragma unroll
+            for (int inner = 0; inner < (mode_special(MODE) ? (GS >> (i + 1)) : (1 << i)); inner++) {
+                const int j0 = mode_special(MODE) ? ((inner << (i + 1)) | outer)
+                                                  : ((outer << (i + 1)) | inner);
+                const int j1 = j0 | (1 << i);
+                cplx<T> U = v[j0], V = v[j1];
+                if constexpr (mode_is_ct(MODE)) {
+                    cplx<T> Vw = V * w;
+                    v[j0] = U + Vw;
+                    v[j1] = U - Vw;
+                } else {
+                    v[j0] = U + V;
+                    v[j1] = (U - V) * w;
+                }
+            }
+        }
+    }
+}
+
+// Group geometry of thread `tid`, k-th group, for a step over tile-row bits
+// [q, q+RHO): the group's rows are rbase | (j << q).
+template <int RHO, int NT>
+__device__ __forceinline__ void group_of(int tid, int k, int logC, int q, int& rbase, int& col)
+{
+    const int gamma = tid + k * NT;
+    col = gamma & ((1 << logC) - 1);
+    const int rest = gamma >> logC;
+    const int rlo = rest & ((1 << q) - 1);
+    const int rhi = rest >> q;
+    rbase = (rhi << (q + RHO)) | rlo;
+}
+
 // One radix-2^RHO step over tile-row bits [q, q+RHO): every thread takes
 // EPT/2^RHO groups of 2^RHO elements from LDS, runs the RHO stages in
 // registers and writes them back.
@@ -126,86 +180,63 @@ template <typename T, int MODE, int E, int RHO, int NT>
 __device__ __forceinline__ void radix_step(cplx<T>* lds, const PassArgs<T>& a, const TileGeo& g,
                                            int q, int tid)
 {
-    constexpr int EPT = 1 << E;
-    constexpr int GS = 1 << RHO;       // group size
-    constexpr int GPT = EPT / GS;      // groups per thread
-    const int logC = g.logC;
-    const int C = 1 << logC;
+    constexpr int GS = 1 << RHO;
+    constexpr int GPT = (1 << E) / GS;
 #pragma unroll
     for (int k = 0; k < GPT; k++) {
-        int gamma = tid + k * NT;
-        int col = gamma & (C - 1);
-        int rest = gamma >> logC;
-        int rlo = rest & ((1 << q) - 1);
-        int rhi = rest >> q;
-        int rbase = (rhi << (q + RHO)) | rlo;
+        int rbase, col;
+        group_of<RHO, NT>(tid, k, g.logC, q, rbase, col);
         cplx<T> v[GS];
-        int li[GS];
 #pragma unroll
-        for (int j = 0; j < GS; j++) {
-            int r = rbase | (j << q);
-            li[j] = swz((r << logC) | col);
-            v[j] = lds[li[j]];
-        }
+        for (int j = 0; j < GS; j++) v[j] = lds[swz(((rbase | (j << q)) << g.logC) | col)];
+        group_stages<T, MODE, RHO>(v, a, g, rbase, col, q);
 #pragma unroll
-        for (int st = 0; st < RHO; st++) {
-            // bit inside the group handled by this stage
-            const int i = mode_descending(MODE) ? (RHO - 1 - st) : st;
-            const int bpos = g.blo + q + i;  // position bit
-            // Standard twiddles depend only on the group bits above i, special
-            // ones only on the bits below i: load each distinct one once.
-#pragma unroll
-            for (int outer = 0; outer < (mode_special(MODE) ? (1 << i) : (GS >> (i + 1))); outer++) {
-                const int j_first = mode_special(MODE) ? outer : (outer << (i + 1));
-                int p0 = g.pos(rbase | (j_first << q), col);
-                cplx<T> w = ldc(a.tab + twiddle_index<T, MODE>(a.n, a.nega, bpos, p0));
-#pragma unroll
-                for (int inner = 0; inner < (mode_special(MODE) ? (GS >> (i + 1)) : (1 << i)); inner++) {
-                    const int j0 = mode_special(MODE) ? ((inner << (i + 1)) | outer)
-                                                      : ((outer << (i + 1)) | inner);
-                    const int j1 = j0 | (1 << i);
-                    cplx<T> U = v[j0], V = v[j1];
-                    if constexpr (mode_is_ct(MODE)) {
-                        cplx<T> Vw = V * w;
-                        v[j0] = U + Vw;
-                        v[j1] = U - Vw;
-                    } else {
-                        v[j0] = U + V;
-                        v[j1] = (U - V) * w;
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < GS; j++) lds[li[j]] = v[j];
+        for (int j = 0; j < GS; j++) lds[swz(((rbase | (j << q)) << g.logC) | col)] = v[j];
     }
 }
 
-template <typename T, int MODE, int E, int NT>
-__device__ __forceinline__ void run_steps(cplx<T>* lds, const PassArgs<T>& a, const TileGeo& g,
-                                          int tid)
+// Bit range of step s (0-based, in execution order) of a pass of K bits made
+// of radix-2^E steps: the one short step sits at the top end of the range,
+// so a descending chain runs it first and an ascending chain last, and the
+// last step of a descending chain covers the same bits [0, E) as the first
+// step of the ascending chain that inverts it.
+__host__ __device__ __forceinline__ void step_range(bool descending, int K, int E, int s, int& q,
+                                                    int& rho)
 {
-    // K stages as steps of E bits plus one remainder step; descending modes
-    // start from the top tile-row bits, ascending modes from bit 0.
-    const int K = a.K;
     const int full = K / E;
     const int rem = K - full * E;
-    for (int s = 0; s < full + (rem ? 1 : 0); s++) {
-        int rho = (s < full) ? E : rem;
-        int q;
-        if (mode_descending(MODE)) {
-            q = K - s * E - rho;  // steps cover [K-E, K), [K-2E, K-E), ..., [0, rem)
-        } else {
-            q = s * E;            // steps cover [0, E), [E, 2E), ..., [full*E, K)
-        }
+    const int nsteps = full + (rem ? 1 : 0);
+    const int as = descending ? (nsteps - 1 - s) : s;  // index in ascending order
+    q = as * E;
+    rho = (as < full) ? E : rem;
+}
+
+__host__ __device__ __forceinline__ int num_steps(int K, int E) { return (K + E - 1) / E; }
+
+template <typename T, int MODE, int E, int NT>
+__device__ __forceinline__ void radix_step_rt(cplx<T>* lds, const PassArgs<T>& a, const TileGeo& g,
+                                              int q, int rho, int tid)
+{
+    switch (rho) {
+        case 1: radix_step<T, MODE, E, 1, NT>(lds, a, g, q, tid); break;
+        case 2: if constexpr (E >= 2) radix_step<T, MODE, E, 2, NT>(lds, a, g, q, tid); break;
+        case 3: if constexpr (E >= 3) radix_step<T, MODE, E, 3, NT>(lds, a, g, q, tid); break;
+        case 4: if constexpr (E >= 4) radix_step<T, MODE, E, 4, NT>(lds, a, g, q, tid); break;
+        case 5: if constexpr (E >= 5) radix_step<T, MODE, E, 5, NT>(lds, a, g, q, tid); break;
+        default: break;
+    }
+}
+
+// Runs steps [s_begin, s_end) of the pass, each between two barriers.
+template <typename T, int MODE, int E, int NT>
+__device__ __forceinline__ void run_steps(cplx<T>* lds, const PassArgs<T>& a, const TileGeo& g,
+                                          int tid, int s_begin, int s_end)
+{
+    for (int s = s_begin; s < s_end; s++) {
+        int q, rho;
+        step_range(mode_descending(MODE), a.K, E, s, q, rho);
         __syncthreads();
-        switch (rho) {
-            case 1: radix_step<T, MODE, E, 1, NT>(lds, a, g, q, tid); break;
-            case 2: if constexpr (E >= 2) radix_step<T, MODE, E, 2, NT>(lds, a, g, q, tid); break;
-            case 3: if constexpr (E >= 3) radix_step<T, MODE, E, 3, NT>(lds, a, g, q, tid); break;
-            case 4: if constexpr (E >= 4) radix_step<T, MODE, E, 4, NT>(lds, a, g, q, tid); break;
-            default: break;
-        }
+        radix_step_rt<T, MODE, E, NT>(lds, a, g, q, rho, tid);
     }
     __syncthreads();
 }
@@ -226,6 +257,7 @@ __global__ __launch_bounds__((1 << LOGT) >> E) void pass_kernel(PassArgs<T> a)
     g.blo = a.blo;
     g.K = a.K;
     g.logC = logC;
+    g.tab = make_rsrc(a.tab);
 
     // tile -> (transform block, hi bits above the pass, column block)
     const int ncb_log = a.blo - logC;
@@ -253,7 +285,7 @@ __global__ __launch_bounds__((1 << LOGT) >> E) void pass_kernel(PassArgs<T> a)
         lds[swz(idx)] = v;
     }
 
-    run_steps<T, MODE, E, NT>(lds, a, g, tid);
+    run_steps<T, MODE, E, NT>(lds, a, g, tid, 0, num_steps(a.K, E));
 
     // ---- store
 #pragma unroll

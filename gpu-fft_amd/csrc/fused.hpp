@@ -1,0 +1,328 @@
+// Fused polynomial-product kernels for gfx950.
+//
+// A product of two length-2^n transforms (FFNT folding: n = log2(N) - 1) is
+// computed by 2^SB workgroups per product.  Workgroup q owns the sub-problem
+// of positions [q 2^S, (q+1) 2^S), S = n - SB: after the top SB Cooley-Tukey
+// stages (bits n-1 .. S) the remaining S stages, the pointwise product and
+// the first S Gentleman-Sande inverse stages of those positions depend on
+// nothing else.  So the workgroup
+//   1. gathers, for each of its 2^S positions c, the 2^SB inputs
+//      j 2^S + c (fold + twist fused) and evaluates row q of the top
+//      SB-stage network in registers (the other workgroups of the product
+//      read the same inputs, which the XCD-aware block order keeps in L2),
+//   2. runs the S forward stages of operand b in LDS and keeps the result in
+//      registers, then does the same for operand a (or each limb of a),
+//   3. multiplies in registers, inverse-transforms in LDS and writes the
+//      2^S partial results to a scratch buffer.
+// A second, streaming pass applies the top SB inverse stages, the scale, the
+// untwist and the unfold (FP64: pass_kernel; FP32 limbs: limb_combine_kernel,
+// which also rounds each limb product and recombines them in int64).  With
+// SB = 0 the whole product is one workgroup and writes the output directly.
+//
+// The stage/twiddle semantics are those of the reference GPU_FFNT
+// (src/lib/fft.cu:1013-1384) and GPU_FFT (fft.cu:11-260): bit-reversed
+// spectra, table index (p >> (b+1)) (+2^(n-1-b) for X^N+1).
+#pragma once
+
+#include "engine.hpp"
+
+namespace gfa {
+
+enum ProdLoad {
+    PL_FOLD = 0,       // T inputs [batch][2^(n+1)]: fold + twist
+    PL_FOLD_LIMB = 1,  // int32 inputs: balanced base-2^w digit, fold + twist
+};
+
+template <typename T> struct ProdArgs {
+    int n;      // log2 of the complex transform length L
+    int S;      // log2 of the sub-problem length
+    int batch;  // products in this launch
+    int nega;
+    const cplx<T>* tab;    // forward root table (reference layout)
+    const cplx<T>* itab;   // inverse root table
+    const cplx<T>* twist;  // FFNT twist table (L entries)
+    const cplx<T>* untwist;
+    const void* a;
+    const void* b;
+    long long in_stride;   // input elements per product
+    cplx<T>* scratch;      // [batch][ka*kb][L] partial results (SB > 0)
+    T* out;                // direct output (SB == 0, PL_FOLD)
+    long long out_stride;
+    cplx<T> scale;
+    int wa, ka, wb, kb;    // limb split of a and b (PL_FOLD_LIMB)
+};
+
+// Balanced base-2^w digit i of v; the top digit (i == k-1) keeps the rest.
+__device__ __forceinline__ int limb_digit(long long v, int w, int i, int k)
+{
+    for (int t = 0; t < i; t++) {
+        long long d = v & ((1LL << w) - 1);
+        if (d >= (1LL << (w - 1))) d -= (1LL << w);
+        v = (v - d) >> w;
+    }
+    if (i == k - 1) return (int)v;
+    long long d = v & ((1LL << w) - 1);
+    if (d >= (1LL << (w - 1))) d -= (1LL << w);
+    return (int)d;
+}
+
+// Folded, twisted input at position P (x: this product's operand, tw: twist).
+template <typename T, int PL>
+__device__ __forceinline__ cplx<T> prod_input(rsrc_t x, rsrc_t tw, int L, int w, int limb, int k,
+                                              int P)
+{
+    cplx<T> z;
+    if constexpr (PL == PL_FOLD) {
+        const unsigned o0 = (unsigned)P * sizeof(T), o1 = (unsigned)(P + L) * sizeof(T);
+        if constexpr (sizeof(T) == 8) {
+            z.x = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(x, o0, 0, 0));
+            z.y = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(x, o1, 0, 0));
+        } else {
+            z.x = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(x, o0, 0, 0));
+            z.y = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(x, o1, 0, 0));
+        }
+    } else {
+        const int v0 = (int)__builtin_amdgcn_raw_buffer_load_b32(x, (unsigned)P * 4u, 0, 0);
+        const int v1 = (int)__builtin_amdgcn_raw_buffer_load_b32(x, (unsigned)(P + L) * 4u, 0, 0);
+        z = mk<T>((T)limb_digit(v0, w, limb, k), (T)limb_digit(v1, w, limb, k));
+    }
+    return z * bldc<T>(tw, P);
+}
+
+// Row q of the top SB forward stages (bits n-1 .. S) applied to zz[j]
+// = input at position j 2^S + c.  The twiddles do not depend on c.
+template <typename T, int SB>
+__device__ __forceinline__ cplx<T> top_forward_row(cplx<T>* zz, const ProdArgs<T>& a, int q)
+{
+    constexpr int Q = 1 << SB;
+#pragma unroll
+    for (int st = 0; st < SB; st++) {
+        const int i = SB - 1 - st;
+        const int b = a.S + i;
+        const int base = a.nega ? (1 << (a.n - 1 - b)) : 0;
+#pragma unroll
+        for (int j0 = 0; j0 < Q; j0++) {
+            if (j0 & (1 << i)) continue;
+            const int j1 = j0 | (1 << i);
+            const cplx<T> w = ldc(a.tab + base + (j0 >> (i + 1)));
+            cplx<T> U = zz[j0], V = zz[j1] * w;
+            zz[j0] = U + V;
+            zz[j1] = U - V;
+        }
+    }
+    cplx<T> r = zz[0];
+#pragma unroll
+    for (int j = 1; j < Q; j++)
+        if (j == q) r = zz[j];
+    return r;
+}
+
+template <typename T, int S, int SB, int E, int PL>
+__device__ __forceinline__ void load_sub(cplx<T>* lds, const ProdArgs<T>& a, rsrc_t x, rsrc_t tw,
+                                         int w, int limb, int k, int q, int tid)
+{
+    constexpr int NT = (1 << S) >> E;
+    constexpr int Q = 1 << SB;
+    const int L = 1 << a.n;
+#pragma unroll 1
+    for (int i = 0; i < (1 << E); i++) {
+        const int c = tid + i * NT;
+        cplx<T> zz[Q];
+#pragma unroll
+        for (int j = 0; j < Q; j++) zz[j] = prod_input<T, PL>(x, tw, L, w, limb, k, (j << S) | c);
+        lds[swz(c)] = top_forward_row<T, SB>(zz, a, q);
+    }
+}
+
+// Last forward step (bits [0, E)) of the thread's contiguous group, into v.
+template <typename T, int E>
+__device__ __forceinline__ void last_forward_step(const cplx<T>* lds, const PassArgs<T>& pf,
+                                                  const TileGeo& g, int tid, cplx<T>* v)
+{
+    const int rbase = tid << E;
+#pragma unroll
+    for (int j = 0; j < (1 << E); j++) v[j] = lds[swz(rbase | j)];
+    group_stages<T, MODE_STD_FWD, E>(v, pf, g, rbase, 0, 0);
+}
+
+template <typename T, int S, int SB, int E, int PL>
+__global__ __launch_bounds__((1 << S) >> E) void product_sub_kernel(ProdArgs<T> a)
+{
+    constexpr int TILE = 1 << S;
+    constexpr int EPT = 1 << E;
+    constexpr int NT = TILE / EPT;
+    constexpr int Q = 1 << SB;
+    __shared__ cplx<T> lds[TILE];
+
+    const int tid = threadIdx.x;
+    // XCD-aware order: consecutive workgroups go to consecutive XCDs, so the
+    // Q workgroups of one product are given block ids congruent mod 8 and
+    // dispatched in consecutive rounds, i.e. on one XCD, close in time.
+    const int x = blockIdx.x;
+    const int xcd = x & 7;
+    const int slot = x >> 3;
+    const int q = slot % Q;
+    const long long p = (long long)(slot / Q) * 8 + xcd;
+    if (p >= a.batch) return;
+
+    PassArgs<T> pf{};
+    pf.n = a.n;
+    pf.K = S;
+    pf.nega = a.nega;
+    pf.tab = a.tab;
+    PassArgs<T> pinv = pf;
+    pinv.tab = a.itab;
+    TileGeo g;
+    g.hi_pos = q << S;
+    g.blo = 0;
+    g.K = S;
+    g.logC = 0;
+    g.tab = make_rsrc(a.tab);
+    TileGeo gi = g;
+    gi.tab = make_rsrc(a.itab);
+    const int nst = num_steps(S, E);
+    const int nprod = a.ka * a.kb;
+    const long long L = 1LL << a.n;
+
+    const rsrc_t xa = make_rsrc(reinterpret_cast<const char*>(a.a) +
+                                p * a.in_stride * (PL == PL_FOLD ? (long long)sizeof(T) : 4LL));
+    const rsrc_t xb = make_rsrc(reinterpret_cast<const char*>(a.b) +
+                                p * a.in_stride * (PL == PL_FOLD ? (long long)sizeof(T) : 4LL));
+    const rsrc_t tw = make_rsrc(a.twist);
+    // compile-time single limb for the FP64 form
+    const int KA = PL == PL_FOLD ? 1 : a.ka;
+    const int KB = PL == PL_FOLD ? 1 : a.kb;
+
+    cplx<T> breg[EPT];
+    for (int jb = 0; jb < KB; jb++) {
+        // an opaque copy of tid per iteration keeps the address arithmetic of
+        // the loop body inside the loop (hoisted, it would hold registers)
+        int t = tid;
+        asm volatile("" : "+v"(t));
+        if (jb) __syncthreads();
+        load_sub<T, S, SB, E, PL>(lds, a, xb, tw, a.wb, jb, KB, q, t);
+        run_steps<T, MODE_STD_FWD, E, NT>(lds, pf, g, t, 0, nst - 1);
+        last_forward_step<T, E>(lds, pf, g, t, breg);
+        for (int ia = 0; ia < KA; ia++) {
+            int u = tid;
+            asm volatile("" : "+v"(u));
+            __syncthreads();
+            load_sub<T, S, SB, E, PL>(lds, a, xa, tw, a.wa, ia, KA, q, u);
+            run_steps<T, MODE_STD_FWD, E, NT>(lds, pf, g, u, 0, nst - 1);
+            cplx<T> v[EPT];
+            last_forward_step<T, E>(lds, pf, g, u, v);
+#pragma unroll
+            for (int j = 0; j < EPT; j++) v[j] = v[j] * breg[j];
+            const int rbase = u << E;
+            group_stages<T, MODE_STD_INV, E>(v, pinv, gi, rbase, 0, 0);
+#pragma unroll
+            for (int j = 0; j < EPT; j++) lds[swz(rbase | j)] = v[j];
+            run_steps<T, MODE_STD_INV, E, NT>(lds, pinv, gi, u, 1, nst);
+            if constexpr (SB == 0 && PL == PL_FOLD) {
+                // whole product in this workgroup: scale, untwist, unfold
+                T* o = a.out + p * a.out_stride;
+                const rsrc_t utw = make_rsrc(a.untwist);
+#pragma unroll 4
+                for (int i = 0; i < EPT; i++) {
+                    const int c = u + i * NT;
+                    cplx<T> z = (lds[swz(c)] * a.scale) * bldc<T>(utw, c);
+                    o[c] = z.x;
+                    o[c + L] = z.y;
+                }
+            } else {
+                cplx<T>* dst = a.scratch + (p * nprod + ia * KB + jb) * L + ((long long)q << S);
+#pragma unroll 4
+                for (int i = 0; i < EPT; i++) {
+                    const int c = u + i * NT;
+                    stc(dst + c, lds[swz(c)]);
+                }
+            }
+        }
+    }
+    (void)Q;
+}
+
+// FP32 limb products -> int64: top SB inverse stages, scale, untwist, unfold,
+// round every limb product and add it at weight 2^(wa*ia + wb*jb).
+// One thread per (product, sub-problem position c).  err (optional) receives
+// per product the largest |x - rint(x)| (as double bits) and the sum of
+// squares over all limb outputs.
+template <int SB> struct CombineArgs {
+    int n, S, batch, nprod, kb, wa, wb;
+    const cplx<float>* scratch;
+    const cplx<float>* itab;
+    const cplx<float>* untwist;
+    float scale;
+    long long* out;
+    long long out_stride;
+    double* err;
+};
+
+template <int SB>
+__global__ __launch_bounds__(256) void limb_combine_kernel(CombineArgs<SB> a)
+{
+    constexpr int Q = 1 << SB;
+    const long long gid = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long p = gid >> a.S;
+    const int c = (int)(gid & ((1LL << a.S) - 1));
+    if (p >= a.batch) return;
+    const long long L = 1LL << a.n;
+    long long acc[Q][2];
+#pragma unroll
+    for (int j = 0; j < Q; j++) acc[j][0] = acc[j][1] = 0;
+    float emax = 0.f;
+    float esq = 0.f;
+    for (int l = 0; l < a.nprod; l++) {
+        const cplx<float>* src = a.scratch + (p * a.nprod + l) * L;
+        cplx<float> y[Q];
+#pragma unroll
+        for (int j = 0; j < Q; j++) y[j] = ldc(src + ((long long)j << a.S) + c);
+#pragma unroll
+        for (int i = 0; i < SB; i++) {  // Gentleman-Sande, bits S .. n-1
+#pragma unroll
+            for (int j0 = 0; j0 < Q; j0++) {
+                if (j0 & (1 << i)) continue;
+                const int j1 = j0 | (1 << i);
+                const cplx<float> w = ldc(a.itab + (j0 >> (i + 1)));
+                cplx<float> U = y[j0], V = y[j1];
+                y[j0] = U + V;
+                y[j1] = (U - V) * w;
+            }
+        }
+        const int shift = a.wa * (l / a.kb) + a.wb * (l % a.kb);
+#pragma unroll
+        for (int j = 0; j < Q; j++) {
+            const int P = (j << a.S) | c;
+            cplx<float> z = mk<float>(y[j].x * a.scale, y[j].y * a.scale) * ldc(a.untwist + P);
+            const float r0 = rintf(z.x), r1 = rintf(z.y);
+            const float e0 = fabsf(z.x - r0), e1 = fabsf(z.y - r1);
+            emax = fmaxf(emax, fmaxf(e0, e1));
+            esq += e0 * e0 + e1 * e1;
+            acc[j][0] += (long long)r0 * (1LL << shift);
+            acc[j][1] += (long long)r1 * (1LL << shift);
+        }
+    }
+    long long* o = a.out + p * a.out_stride;
+#pragma unroll
+    for (int j = 0; j < Q; j++) {
+        const int P = (j << a.S) | c;
+        o[P] = acc[j][0];
+        o[P + L] = acc[j][1];
+    }
+    if (a.err) {
+        // one wavefront holds 64 consecutive positions of one product
+        double dmax = emax, dsq = esq;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            dmax = fmax(dmax, __shfl_xor(dmax, off, 64));
+            dsq += __shfl_xor(dsq, off, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicMax(reinterpret_cast<unsigned long long*>(a.err + 2 * p),
+                      (unsigned long long)__double_as_longlong(dmax));
+            atomicAdd(a.err + 2 * p + 1, dsq);
+        }
+    }
+}
+
+}  // namespace gfa

@@ -1,0 +1,336 @@
+// Fused batched polynomial multiplication (additive API; the reference needs
+// two GPU_FFNT calls and a caller-side rounding for the same product,
+// example/test_ffnt_gpu_R_R.cu:137-162).
+//
+// A plan owns its root tables (generated on the GPU in FP64 in the
+// reference's FFNT layouts) and the scratch buffer of the split kernels.
+//   GPUFFT_PREC_F64        double in, double out (unrounded, like GPU_FFNT)
+//   GPUFFT_PREC_F32_SPLIT  int32 in, int64 out: the operands are split into
+//                          balanced base-2^w limbs, every limb product runs in
+//                          FP32 and is rounded, and the rounded limb products
+//                          are recombined exactly in int64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+
+#include "fused.hpp"
+#include "status.hpp"
+
+using namespace gfa;
+
+struct gpufft_polymul_plan_s {
+    int device;
+    int ring_power;  // log2 N
+    int n;           // log2 L, L = N/2
+    int precision;
+    int max_batch;
+    int wa, ka, wb, kb;
+    int S, SB;       // sub-problem split
+    void* root;      // forward FFNT root table (L entries)
+    void* iroot;
+    void* twist;
+    void* untwist;
+    void* scratch;
+    size_t scratch_bytes;
+};
+
+namespace {
+
+constexpr int F64_MAX_S = 13;  // 2^13 doubles complex = 128 KiB of LDS
+constexpr int F32_MAX_S = 14;  // 2^14 floats complex = 128 KiB of LDS
+constexpr int MAX_SB = 3;
+
+template <typename T> using ProdKernel = void (*)(ProdArgs<T>);
+
+// Zero fill by a kernel rather than hipMemsetAsync, so that a caller may
+// capture the call into a HIP graph.
+__global__ void zero_kernel(double* p, long long n)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+template <typename T, int PL> ProdKernel<T> prod_kernel(int S, int SB)
+{
+    // E = 4 (double) / 5 (float): 512 threads per 128 KiB tile
+    constexpr int E = sizeof(T) == 8 ? 4 : 5;
+    constexpr int SMAX = sizeof(T) == 8 ? F64_MAX_S : F32_MAX_S;
+#define GFA_PK(SS, BB) \
+    if (S == SS && SB == BB) return &product_sub_kernel<T, SS, BB, E, PL>;
+    if (SB > 0) {
+        GFA_PK(SMAX, 1)
+        GFA_PK(SMAX, 2)
+        GFA_PK(SMAX, 3)
+    } else {
+        GFA_PK(6, 0)
+        GFA_PK(7, 0)
+        GFA_PK(8, 0)
+        GFA_PK(9, 0)
+        GFA_PK(10, 0)
+        GFA_PK(11, 0)
+        GFA_PK(12, 0)
+        GFA_PK(13, 0)
+        if constexpr (SMAX >= 14) {
+            GFA_PK(14, 0)
+        }
+    }
+#undef GFA_PK
+    return nullptr;
+}
+
+template <typename T> int make_tables(gpufft_polymul_plan_s* pl, hipStream_t st)
+{
+    size_t bytes = sizeof(cplx<T>) << pl->n;
+    void** dst[4] = {&pl->root, &pl->iroot, &pl->twist, &pl->untwist};
+    for (void** d : dst) {
+        hipError_t e = hipMalloc(d, bytes);
+        if (e != hipSuccess) return fail_hip(e, "gpufft_polymul_plan_create: hipMalloc");
+    }
+    int (*gen)(void*, int, int, int, void*) =
+        sizeof(T) == 8 ? gpufft_generate_table_f64 : gpufft_generate_table_f32;
+    int s;
+    if ((s = gen(pl->root, GPUFFT_TABLE_FFNT_ROOT, pl->n, 0, st))) return s;
+    if ((s = gen(pl->iroot, GPUFFT_TABLE_FFNT_ROOT, pl->n, 1, st))) return s;
+    if ((s = gen(pl->twist, GPUFFT_TABLE_FFNT_TWIST, pl->n, 0, st))) return s;
+    if ((s = gen(pl->untwist, GPUFFT_TABLE_FFNT_TWIST, pl->n, 1, st))) return s;
+    return GPUFFT_OK;
+}
+
+void release(gpufft_polymul_plan_s* pl)
+{
+    if (!pl) return;
+    void* ptrs[5] = {pl->root, pl->iroot, pl->twist, pl->untwist, pl->scratch};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete pl;
+}
+
+template <typename T, int PL>
+int launch_products(const gpufft_polymul_plan_s* pl, const void* a, const void* b, T* out_direct,
+                    int batch, hipStream_t st, const char* who)
+{
+    ProdArgs<T> pa{};
+    pa.n = pl->n;
+    pa.S = pl->S;
+    pa.batch = batch;
+    pa.nega = 0;
+    pa.tab = (const cplx<T>*)pl->root;
+    pa.itab = (const cplx<T>*)pl->iroot;
+    pa.twist = (const cplx<T>*)pl->twist;
+    pa.untwist = (const cplx<T>*)pl->untwist;
+    pa.a = a;
+    pa.b = b;
+    pa.in_stride = 2LL << pl->n;
+    pa.scratch = (cplx<T>*)pl->scratch;
+    pa.out = out_direct;
+    pa.out_stride = 2LL << pl->n;
+    pa.scale = mk<T>((T)(1.0 / (double)(1LL << pl->n)), (T)0);
+    pa.wa = pl->wa;
+    pa.ka = pl->ka;
+    pa.wb = pl->wb;
+    pa.kb = pl->kb;
+    ProdKernel<T> k = prod_kernel<T, PL>(pl->S, pl->SB);
+    if (!k) return fail(GPUFFT_ERR_UNSUPPORTED, std::string(who) + ": no kernel for this size");
+    constexpr int E = sizeof(T) == 8 ? 4 : 5;
+    const int threads = (1 << pl->S) >> E;
+    const long long groups = ((long long)batch + 7) / 8;
+    const long long blocks = groups * 8 * (1LL << pl->SB);
+    hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(threads), 0, st, pa);
+    GFA_CHECK_LAUNCH(who);
+    return GPUFFT_OK;
+}
+
+int check_plan(const gpufft_polymul_plan_s* pl, int precision, int batch, const char* who)
+{
+    if (!pl) return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": plan is NULL");
+    if (pl->precision != precision)
+        return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": plan precision mismatch");
+    if (batch < 1) return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": batch must be >= 1");
+    return GPUFFT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gpufft_polymul_plan_create(gpufft_polymul_plan* out, int ring_power, int precision,
+                               int max_batch, int a_limb_bits, int a_limbs, int b_limb_bits,
+                               int b_limbs, void* stream)
+{
+    const char* who = "gpufft_polymul_plan_create";
+    if (!out) return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": out is NULL");
+    *out = nullptr;
+    if (precision != GPUFFT_PREC_F64 && precision != GPUFFT_PREC_F32_SPLIT)
+        return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": unknown precision");
+    const int E = precision == GPUFFT_PREC_F64 ? 4 : 5;
+    const int smax = precision == GPUFFT_PREC_F64 ? F64_MAX_S : F32_MAX_S;
+    const int n = ring_power - 1;
+    if (n < E + 2 || n > smax + MAX_SB)
+        return fail(GPUFFT_ERR_INVALID_ARGUMENT,
+                    std::string(who) + ": ring_power out of the fused range");
+    if (max_batch < 1) return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": max_batch < 1");
+    if (precision == GPUFFT_PREC_F32_SPLIT) {
+        if (a_limbs < 1 || b_limbs < 1 || a_limbs * b_limbs > 64 ||
+            (a_limbs > 1 && (a_limb_bits < 1 || a_limb_bits > 16)) ||
+            (b_limbs > 1 && (b_limb_bits < 1 || b_limb_bits > 16)))
+            return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": invalid limb split");
+    }
+    gpufft_polymul_plan_s* pl = new gpufft_polymul_plan_s();
+    (void)hipGetDevice(&pl->device);
+    pl->ring_power = ring_power;
+    pl->n = n;
+    pl->precision = precision;
+    pl->max_batch = max_batch;
+    if (precision == GPUFFT_PREC_F32_SPLIT) {
+        pl->wa = a_limbs > 1 ? a_limb_bits : 0;
+        pl->ka = a_limbs;
+        pl->wb = b_limbs > 1 ? b_limb_bits : 0;
+        pl->kb = b_limbs;
+    } else {
+        pl->ka = pl->kb = 1;
+    }
+    pl->S = std::min(n, smax);
+    pl->SB = n - pl->S;
+    hipStream_t st = (hipStream_t)stream;
+    int s = precision == GPUFFT_PREC_F64 ? make_tables<double>(pl, st) : make_tables<float>(pl, st);
+    if (s) {
+        release(pl);
+        return s;
+    }
+    const bool need_scratch = precision == GPUFFT_PREC_F32_SPLIT || pl->SB > 0;
+    if (need_scratch) {
+        size_t elem = precision == GPUFFT_PREC_F64 ? 16 : 8;
+        pl->scratch_bytes = (size_t)max_batch * pl->ka * pl->kb * (elem << n);
+        hipError_t e = hipMalloc(&pl->scratch, pl->scratch_bytes);
+        if (e != hipSuccess) {
+            release(pl);
+            return fail_hip(e, "gpufft_polymul_plan_create: hipMalloc scratch");
+        }
+    }
+    *out = pl;
+    return ok();
+}
+
+int gpufft_polymul_plan_destroy(gpufft_polymul_plan plan)
+{
+    release(plan);
+    return GPUFFT_OK;
+}
+
+int gpufft_polymul_f64(gpufft_polymul_plan plan, const double* a, const double* b, double* out,
+                       int batch, void* stream)
+{
+    return gpufft_polymul_f64_phases(plan, a, b, out, batch, GPUFFT_PHASE_ALL, stream);
+}
+
+int gpufft_polymul_f64_phases(gpufft_polymul_plan plan, const double* a, const double* b,
+                              double* out, int batch, int phases, void* stream)
+{
+    const char* who = "gpufft_polymul_f64";
+    if (int s = check_plan(plan, GPUFFT_PREC_F64, batch, who)) return s;
+    if (!a || !b || !out) return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": NULL buffer");
+    hipStream_t st = (hipStream_t)stream;
+    const long long N = 2LL << plan->n;
+    for (int b0 = 0; b0 < batch; b0 += plan->max_batch) {
+        const int nb = std::min(plan->max_batch, batch - b0);
+        const double* pa = a + b0 * N;
+        const double* pb = b + b0 * N;
+        double* po = out + b0 * N;
+        if (phases & GPUFFT_PHASE_TRANSFORM)
+            if (int s = launch_products<double, PL_FOLD>(plan, pa, pb, po, nb, st, who)) return s;
+        if (plan->SB > 0 && (phases & GPUFFT_PHASE_COMBINE)) {
+            // top SB inverse stages + scale + untwist + unfold
+            PassArgs<double> pa2{};
+            pa2.n = plan->n;
+            pa2.blo = plan->S;
+            pa2.K = plan->SB;
+            pa2.logC = 12 - plan->SB;
+            pa2.logBT = 0;
+            pa2.batch = nb;
+            pa2.nega = 0;
+            pa2.do_scale = 1;
+            pa2.scale = mk<double>(1.0 / (double)(1LL << plan->n), 0.0);
+            pa2.tab = (const cplx<double>*)plan->iroot;
+            pa2.src = plan->scratch;
+            pa2.src_stride = 1LL << plan->n;
+            pa2.dst = po;
+            pa2.stab = (const cplx<double>*)plan->untwist;
+            pa2.dst_stride = N;
+            const long long tiles = ((long long)nb << plan->n) >> 12;
+            hipLaunchKernelGGL((pass_kernel<double, MODE_STD_INV, 12, 4, LD_CPLX, ST_UNFOLD>),
+                               dim3((unsigned)tiles), dim3(256), 0, st, pa2);
+            GFA_CHECK_LAUNCH(who);
+        }
+    }
+    return ok();
+}
+
+int gpufft_polymul_i32(gpufft_polymul_plan plan, const int32_t* a, const int32_t* b, int64_t* out,
+                       int batch, double* err_stats, void* stream)
+{
+    return gpufft_polymul_i32_phases(plan, a, b, out, batch, err_stats, GPUFFT_PHASE_ALL, stream);
+}
+
+int gpufft_polymul_i32_phases(gpufft_polymul_plan plan, const int32_t* a, const int32_t* b,
+                              int64_t* out, int batch, double* err_stats, int phases, void* stream)
+{
+    const char* who = "gpufft_polymul_i32";
+    if (int s = check_plan(plan, GPUFFT_PREC_F32_SPLIT, batch, who)) return s;
+    if (!a || !b || !out) return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": NULL buffer");
+    hipStream_t st = (hipStream_t)stream;
+    const long long N = 2LL << plan->n;
+    if (err_stats && (phases & GPUFFT_PHASE_COMBINE)) {
+        const long long cnt = 2LL * batch;
+        hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st,
+                           err_stats, cnt);
+        GFA_CHECK_LAUNCH(who);
+    }
+    for (int b0 = 0; b0 < batch; b0 += plan->max_batch) {
+        const int nb = std::min(plan->max_batch, batch - b0);
+        if (phases & GPUFFT_PHASE_TRANSFORM)
+            if (int s = launch_products<float, PL_FOLD_LIMB>(plan, a + b0 * N, b + b0 * N, nullptr,
+                                                             nb, st, who))
+                return s;
+        if (!(phases & GPUFFT_PHASE_COMBINE)) continue;
+        const int nthreads_total = nb << plan->S;
+        const unsigned blocks = (unsigned)((nthreads_total + 255) / 256);
+#define GFA_COMB(BB)                                                                       \
+    case BB: {                                                                             \
+        CombineArgs<BB> ca{};                                                              \
+        ca.n = plan->n;                                                                    \
+        ca.S = plan->S;                                                                    \
+        ca.batch = nb;                                                                     \
+        ca.nprod = plan->ka * plan->kb;                                                    \
+        ca.kb = plan->kb;                                                                  \
+        ca.wa = plan->wa;                                                                  \
+        ca.wb = plan->wb;                                                                  \
+        ca.scratch = (const cplx<float>*)plan->scratch;                                    \
+        ca.itab = (const cplx<float>*)plan->iroot;                                         \
+        ca.untwist = (const cplx<float>*)plan->untwist;                                    \
+        ca.scale = (float)(1.0 / (double)(1LL << plan->n));                                \
+        ca.out = (long long*)(out + b0 * N);                                               \
+        ca.out_stride = N;                                                                 \
+        ca.err = err_stats ? err_stats + 2 * b0 : nullptr;                                 \
+        hipLaunchKernelGGL(limb_combine_kernel<BB>, dim3(blocks), dim3(256), 0, st, ca);   \
+        break;                                                                             \
+    }
+        switch (plan->SB) {
+            GFA_COMB(0)
+            GFA_COMB(1)
+            GFA_COMB(2)
+            GFA_COMB(3)
+            default: return fail(GPUFFT_ERR_UNSUPPORTED, std::string(who) + ": split too deep");
+        }
+#undef GFA_COMB
+        GFA_CHECK_LAUNCH(who);
+    }
+    return ok();
+}
+
+size_t gpufft_polymul_plan_scratch_bytes(gpufft_polymul_plan plan)
+{
+    return plan ? plan->scratch_bytes : 0;
+}
+
+}  // extern "C"

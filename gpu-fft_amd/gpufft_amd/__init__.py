@@ -69,6 +69,14 @@ _SIGS = {
     "gpufft_ffnt_f64": [_P, _P, _P, _P, _CP, _I, _I],
     "gpufft_special_fft_f32": [_P, _P, _CP, _I],
     "gpufft_special_fft_f64": [_P, _P, _CP, _I],
+    "gpufft_generate_table_f64": [_P, _I, _I, _I, _P],
+    "gpufft_generate_table_f32": [_P, _I, _I, _I, _P],
+    "gpufft_polymul_plan_create": [_P, _I, _I, _I, _I, _I, _I, _I, _P],
+    "gpufft_polymul_plan_destroy": [_P],
+    "gpufft_polymul_f64": [_P, _P, _P, _P, _I, _P],
+    "gpufft_polymul_i32": [_P, _P, _P, _P, _I, _P, _P],
+    "gpufft_polymul_f64_phases": [_P, _P, _P, _P, _I, _I, _P],
+    "gpufft_polymul_i32_phases": [_P, _P, _P, _P, _I, _P, _I, _P],
 }
 
 
@@ -81,6 +89,8 @@ def _declare():
     _lib.gpufft_last_error_message.restype = ctypes.c_char_p
     _lib.gpufft_last_hip_error.argtypes = []
     _lib.gpufft_last_hip_error.restype = ctypes.c_int
+    _lib.gpufft_polymul_plan_scratch_bytes.argtypes = [_P]
+    _lib.gpufft_polymul_plan_scratch_bytes.restype = ctypes.c_size_t
 
 
 _declare()
@@ -170,3 +180,59 @@ def GPU_Special_FFT(inout, root_table, cfg, batch_size):
     """GPU_Special_FFT (reference fft.cuh:194-197)."""
     fn = getattr(_lib, "gpufft_special_fft_" + _prec(inout))
     check(fn(_ptr(inout), _ptr(root_table), ctypes.byref(cfg), batch_size))
+
+
+# ---- GPU table generation (reference layouts) ----
+TABLE_FFT, TABLE_FFNT_ROOT, TABLE_FFNT_TWIST, TABLE_SPECIAL = 0, 1, 2, 3
+
+
+def generate_table(kind, n_power, inverse=False, prec="f64", stream=None):
+    """Root-of-unity table of 2^n_power complex entries, built on the GPU."""
+    import torch
+    dt = torch.complex128 if prec == "f64" else torch.complex64
+    out = torch.empty(1 << n_power, dtype=dt, device="cuda")
+    fn = getattr(_lib, "gpufft_generate_table_" + prec)
+    check(fn(out.data_ptr(), kind, n_power, int(inverse), _stream(stream)))
+    return out
+
+
+# ---- fused batched negacyclic products ----
+PREC_F64, PREC_F32_SPLIT = 0, 1
+PHASE_TRANSFORM, PHASE_COMBINE, PHASE_ALL = 1, 2, 3
+
+
+class PolymulPlan:
+    """Plan of gpufft_polymul_* for ring size N = 2^ring_power (mod X^N+1)."""
+
+    def __init__(self, ring_power, precision=PREC_F64, max_batch=256, a_limb_bits=0, a_limbs=1,
+                 b_limb_bits=0, b_limbs=1, stream=None):
+        h = ctypes.c_void_p()
+        check(_lib.gpufft_polymul_plan_create(ctypes.byref(h), ring_power, precision, max_batch,
+                                              a_limb_bits, a_limbs, b_limb_bits, b_limbs,
+                                              _stream(stream)))
+        self.handle = h
+        self.ring_power = ring_power
+        self.precision = precision
+
+    @property
+    def scratch_bytes(self):
+        return _lib.gpufft_polymul_plan_scratch_bytes(self.handle)
+
+    def multiply_f64(self, a, b, out, batch, stream=None, phases=3):
+        check(_lib.gpufft_polymul_f64_phases(self.handle, _ptr(a), _ptr(b), _ptr(out), batch,
+                                             phases, _stream(stream)))
+
+    def multiply_i32(self, a, b, out, batch, err_stats=None, stream=None, phases=3):
+        check(_lib.gpufft_polymul_i32_phases(self.handle, _ptr(a), _ptr(b), _ptr(out), batch,
+                                             _ptr(err_stats), phases, _stream(stream)))
+
+    def close(self):
+        if self.handle:
+            _lib.gpufft_polymul_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

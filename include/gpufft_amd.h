@@ -16,6 +16,7 @@
 #ifndef GPUFFT_AMD_H
 #define GPUFFT_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -102,6 +103,77 @@ int gpufft_special_fft_f32(void* inout, const void* root_table, const gpufft_con
                            int batch);
 int gpufft_special_fft_f64(void* inout, const void* root_table, const gpufft_config* cfg,
                            int batch);
+
+/* ---- root-of-unity tables generated on the GPU (FP64 arithmetic) ----
+ * Fills dst[0 .. 2^n_power) on `stream` in the reference layouts:
+ *   GPUFFT_TABLE_FFT         FFT<T>(2^n_power).ReverseRootTable()
+ *                            (reference src/lib/fft_cpu.cu:93-118,200-211);
+ *                            inverse != 0: InverseReverseRootTable()
+ *   GPUFFT_TABLE_FFNT_ROOT   FFNT<T>(2^(n_power+1)).ReverseRootTable_ffnt()
+ *                            (fft_cpu.cu:251-263,434-446); inverse: _Inverse_
+ *   GPUFFT_TABLE_FFNT_TWIST  FFNT<T>(2^(n_power+1)).twist_table_ffnt()
+ *                            (fft_cpu.cu:276-288,462-465); inverse: untwist
+ *   GPUFFT_TABLE_SPECIAL     special-FFT table of 2^n_power slots,
+ *                            T[h+j] = exp(2 pi i (5^j mod 8h)/8h) (layout
+ *                            indexed at fft.cu:782,908); inverse: conjugate
+ * The reference builds these on the host with libm (same layout and values
+ * to within an ulp); this entry point has no counterpart there. */
+#define GPUFFT_TABLE_FFT 0
+#define GPUFFT_TABLE_FFNT_ROOT 1
+#define GPUFFT_TABLE_FFNT_TWIST 2
+#define GPUFFT_TABLE_SPECIAL 3
+int gpufft_generate_table_f64(void* dst, int kind, int n_power, int inverse, void* stream);
+int gpufft_generate_table_f32(void* dst, int kind, int n_power, int inverse, void* stream);
+
+/* ---- fused batched negacyclic products (no counterpart in the reference,
+ * which multiplies with GPU_FFNT FORWARD + GPU_FFNT INVERSE(multiplication)
+ * and rounds on the caller side, example/test_ffnt_gpu_R_R.cu:137-172) ----
+ * A plan fixes the ring size N = 2^ring_power, the precision and the largest
+ * batch of one launch; it owns its root tables (generated on the GPU) and
+ * its scratch buffer (allocated once at creation).  Execution allocates
+ * nothing; calls on one plan must not overlap in time (they share scratch).
+ *   GPUFFT_PREC_F64       a, b, out: double [batch][N]; out is the unrounded
+ *                         product mod X^N+1 (same values as the two-call
+ *                         GPU_FFNT flow); ring_power 7..17.
+ *   GPUFFT_PREC_F32_SPLIT a, b: int32 [batch][N]; out: int64 [batch][N], the
+ *                         product mod X^N+1.  a is split into a_limbs
+ *                         balanced base-2^a_limb_bits digits (the top one
+ *                         keeps the rest), b into b_limbs digits; each limb
+ *                         product is an FP32 FFNT product rounded to an
+ *                         integer, and the limbs are recombined in int64.
+ *                         err_stats (optional, double [batch][2]) receives
+ *                         per product the largest distance of a limb output
+ *                         to its rounded value and the sum of squares of
+ *                         those distances: a largest distance below 0.5
+ *                         means every rounding was to the exact integer.
+ *                         ring_power 8..18. */
+#define GPUFFT_PREC_F64 0
+#define GPUFFT_PREC_F32_SPLIT 1
+typedef struct gpufft_polymul_plan_s* gpufft_polymul_plan;
+int gpufft_polymul_plan_create(gpufft_polymul_plan* out, int ring_power, int precision,
+                               int max_batch, int a_limb_bits, int a_limbs, int b_limb_bits,
+                               int b_limbs, void* stream);
+int gpufft_polymul_plan_destroy(gpufft_polymul_plan plan);
+size_t gpufft_polymul_plan_scratch_bytes(gpufft_polymul_plan plan);
+int gpufft_polymul_f64(gpufft_polymul_plan plan, const double* a, const double* b, double* out,
+                       int batch, void* stream);
+int gpufft_polymul_i32(gpufft_polymul_plan plan, const int32_t* a, const int32_t* b, int64_t* out,
+                       int batch, double* err_stats, void* stream);
+/* The same products as two separately enqueued phases, for callers that
+ * schedule or time them apart: GPUFFT_PHASE_TRANSFORM runs the forward
+ * transforms, products and inverse sub-transforms into the plan's scratch,
+ * GPUFFT_PHASE_COMBINE the last inverse stages and the output (for plans
+ * whose whole product fits one workgroup the first phase writes the output
+ * and the second does nothing).  A combine must follow the transform of the
+ * same inputs on the same plan, with batch <= max_batch. */
+#define GPUFFT_PHASE_TRANSFORM 1
+#define GPUFFT_PHASE_COMBINE 2
+#define GPUFFT_PHASE_ALL 3
+int gpufft_polymul_f64_phases(gpufft_polymul_plan plan, const double* a, const double* b,
+                              double* out, int batch, int phases, void* stream);
+int gpufft_polymul_i32_phases(gpufft_polymul_plan plan, const int32_t* a, const int32_t* b,
+                              int64_t* out, int batch, double* err_stats, int phases,
+                              void* stream);
 
 /* ---- errors ---- */
 /* Human-readable description of the last failure on the calling thread. */

@@ -61,3 +61,23 @@ def test_null_and_batch_validation():
         G.GPU_FFT(0x1000, 0x2000, _cfg(G, 12, 7), 1)
     st = G.lib().gpufft_fft_c2c_f64(0x1000, 0x2000, None, 1, 0)
     assert st == G.ERR_NULL_POINTER
+
+
+def test_polymul_plan_validation_on_host():
+    """Plan arguments are checked before any device call."""
+    import gpufft_amd as G
+    for args in ((3, G.PREC_F64), (30, G.PREC_F64), (12, 7)):
+        with pytest.raises(ValueError):
+            G.PolymulPlan(*args, stream=0)
+    with pytest.raises(ValueError, match="limb"):
+        G.PolymulPlan(12, G.PREC_F32_SPLIT, 4, 40, 2, 0, 1, stream=0)
+    with pytest.raises(ValueError, match="max_batch"):
+        G.PolymulPlan(12, G.PREC_F64, 0, stream=0)
+
+
+def test_table_generation_validation_on_host():
+    import gpufft_amd as G
+    with pytest.raises(ValueError):
+        G.check(G.lib().gpufft_generate_table_f64(0x1000, 9, 4, 0, None))
+    with pytest.raises(ValueError):
+        G.check(G.lib().gpufft_generate_table_f64(None, 0, 4, 0, None))

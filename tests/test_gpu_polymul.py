@@ -1,0 +1,133 @@
+"""GPU parity of the fused batched negacyclic products (gpufft_polymul_*) and of
+the GPU-generated root tables.
+
+Products are compared bit-exactly (after rounding, for the FP64 form) with
+the oracle's exact integer product (NTT over 2^64-2^32+1, itself pinned to the
+schoolbook product in test_oracle.py); the FP64 form is also compared, before
+rounding, with the two-call GPU_FFNT flow of the reference.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import SEED_BASE, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+
+def _g():
+    import gpufft_amd as G
+    return G
+
+
+def _operands(N, batch, seed, a_bits=20, b_lo=-128, b_hi=129):
+    a = O.fill_uniform(seed, 0, 1 << a_bits, batch * N).reshape(batch, N)
+    b = O.fill_uniform(seed + 0x100, b_lo, b_hi, batch * N).reshape(batch, N)
+    return a, b
+
+
+# ---- tables ----
+@pytest.mark.parametrize("n_power", [1, 2, 6, 12, 15])
+def test_gpu_tables_match_reference_layout(n_power):
+    G = _g()
+    n = 1 << n_power
+    ft = O.fft_tables(n)
+    assert np.max(np.abs(to_host(G.generate_table(G.TABLE_FFT, n_power)) - ft["rev"])) < 5e-16
+    assert np.max(np.abs(to_host(G.generate_table(G.TABLE_FFT, n_power, True)) - ft["inv_rev"])) < 5e-16
+    nt = O.ffnt_tables(2 * n)
+    for kind, inv, key in ((G.TABLE_FFNT_ROOT, False, "rev_root"), (G.TABLE_FFNT_ROOT, True, "inv_rev_root"),
+                           (G.TABLE_FFNT_TWIST, False, "twist"), (G.TABLE_FFNT_TWIST, True, "untwist")):
+        got = to_host(G.generate_table(kind, n_power, inv))
+        assert np.max(np.abs(got - nt[key])) < 5e-16, key
+    for inv in (False, True):
+        got = to_host(G.generate_table(G.TABLE_SPECIAL, n_power, inv))
+        assert np.max(np.abs(got - O.special_table(n, inv))) < 5e-16
+    # float tables: the double values rounded once
+    got32 = to_host(G.generate_table(G.TABLE_FFNT_TWIST, n_power, False, prec="f32"))
+    assert np.array_equal(got32, nt["twist"].astype(np.complex64))
+
+
+# ---- FP64 fused product ----
+@pytest.mark.parametrize("ring_power", [7, 8, 12, 14, 15, 16, 17])
+def test_polymul_f64_exact(ring_power):
+    G = _g()
+    import torch
+    N = 1 << ring_power
+    batch = 11  # not a multiple of the 8-XCD block grouping
+    a, b = _operands(N, batch, SEED_BASE + 3 + ring_power)
+    plan = G.PolymulPlan(ring_power, G.PREC_F64, max_batch=batch)
+    out = torch.empty(batch * N, dtype=torch.float64, device="cuda")
+    plan.multiply_f64(to_dev(a.astype(np.float64).reshape(-1)), to_dev(b.astype(np.float64).reshape(-1)),
+                      out, batch)
+    got = to_host(out).reshape(batch, N)
+    exact = O.ntt_negacyclic(a, b)
+    assert np.array_equal(np.round(got).astype(np.int64), exact)
+    # same unrounded values as the oracle's two-call GPU_FFNT flow, to FP64 rounding
+    t = O.ffnt_tables(N)
+    L = N // 2
+    fwd = O.ffnt_forward(np.concatenate([a, b]).astype(np.float64).reshape(-1), ring_power, 2 * batch,
+                         t["twist"], t["rev_root"])
+    ref = O.ffnt_inverse(fwd, ring_power, batch, t["untwist"], t["inv_rev_root"], 1, complex(1.0 / L))
+    ref = ref.reshape(batch, N)
+    assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_polymul_f64_chunks_and_reuse():
+    """batch larger than max_batch runs in chunks; a second call on the same
+    plan with a smaller batch gives the same products."""
+    G = _g()
+    import torch
+    rp, N = 15, 1 << 15
+    a, b = _operands(N, 20, SEED_BASE + 77)
+    plan = G.PolymulPlan(rp, G.PREC_F64, max_batch=8)
+    da, db = to_dev(a.astype(np.float64).reshape(-1)), to_dev(b.astype(np.float64).reshape(-1))
+    out = torch.empty(20 * N, dtype=torch.float64, device="cuda")
+    plan.multiply_f64(da, db, out, 20)
+    exact = O.ntt_negacyclic(a, b)
+    assert np.array_equal(np.round(to_host(out)).astype(np.int64).reshape(20, N), exact)
+    out2 = torch.empty(3 * N, dtype=torch.float64, device="cuda")
+    plan.multiply_f64(da, db, out2, 3)
+    assert np.array_equal(np.round(to_host(out2)).astype(np.int64).reshape(3, N), exact[:3])
+
+
+# ---- FP32-split fused product ----
+SPLITS = {
+    # (a_limb_bits, a_limbs, b_limb_bits, b_limbs)
+    "a5x5": (5, 5, 0, 1),
+    "a4x6": (4, 6, 0, 1),
+    "a7x3_b4x2": (7, 3, 4, 2),
+}
+
+
+@pytest.mark.parametrize("split", sorted(SPLITS))
+@pytest.mark.parametrize("ring_power", [8, 12, 15, 16])
+def test_polymul_f32_split_exact(ring_power, split):
+    G = _g()
+    import torch
+    N = 1 << ring_power
+    batch = 5
+    a, b = _operands(N, batch, SEED_BASE + 33 + ring_power)
+    wa, ka, wb, kb = SPLITS[split]
+    plan = G.PolymulPlan(ring_power, G.PREC_F32_SPLIT, batch, wa, ka, wb, kb)
+    out = torch.empty(batch * N, dtype=torch.int64, device="cuda")
+    err = torch.empty(batch * 2, dtype=torch.float64, device="cuda")
+    plan.multiply_i32(to_dev(a.astype(np.int32).reshape(-1)), to_dev(b.astype(np.int32).reshape(-1)),
+                      out, batch, err)
+    got = to_host(out).reshape(batch, N)
+    e = to_host(err).reshape(batch, 2)
+    exact = O.ntt_negacyclic(a, b)
+    assert np.all(e[:, 0] < 0.5), e
+    assert np.array_equal(got, exact)
+
+
+def test_polymul_f32_unsplit_small_coefficients():
+    """No split: exact while the products stay small (here |c| < 2^14)."""
+    G = _g()
+    import torch
+    rp, N, batch = 12, 1 << 12, 4
+    a, b = _operands(N, batch, SEED_BASE + 5, a_bits=3, b_lo=-4, b_hi=4)
+    plan = G.PolymulPlan(rp, G.PREC_F32_SPLIT, batch, 0, 1, 0, 1)
+    out = torch.empty(batch * N, dtype=torch.int64, device="cuda")
+    plan.multiply_i32(to_dev(a.astype(np.int32).reshape(-1)), to_dev(b.astype(np.int32).reshape(-1)),
+                      out, batch)
+    assert np.array_equal(to_host(out).reshape(batch, N), O.ntt_negacyclic(a, b))
