@@ -38,6 +38,8 @@ def parse():
     p.add_argument("--forms", default="f64,f32split,f64_dropin")
     p.add_argument("--a-limbs", default="5x5", help="a limb split, bits x count")
     p.add_argument("--cpu-seconds", type=float, default=10.0)
+    p.add_argument("--rb-f64", type=int, default=0, help="radix bits of the FP64 kernel (0: tune)")
+    p.add_argument("--rb-f32", type=int, default=0, help="radix bits of the FP32 kernel (0: tune)")
     p.add_argument("--no-cpu", action="store_true")
     return p.parse_args()
 
@@ -95,6 +97,12 @@ def main():
         if "f64" in forms:
             plan = G.PolymulPlan(rp, G.PREC_F64, max_batch=batch, stream=sid)
             out = torch.empty(batch * N, dtype=torch.float64, device=dev)
+            tune = {}
+            for rbv in ([args.rb_f64] if args.rb_f64 else [3, 4]):
+                plan.radix_bits = rbv
+                tune[rbv] = timed(lambda: plan.multiply_f64(a64, b64, out, batch, stream=sid),
+                                  stream, max(3, args.steps // 4), 1)
+            plan.radix_bits = min(tune, key=tune.get)
             ms = timed(lambda: plan.multiply_f64(a64, b64, out, batch, stream=sid), stream,
                        args.steps, args.warmup)
             ms_t = timed(lambda: plan.multiply_f64(a64, b64, out, batch, stream=sid,
@@ -106,6 +114,7 @@ def main():
             r = torch.round(out)
             d = (out - r).abs()
             res["f64"] = {"ms": ms, "ms_transform": ms_t, "ms_combine": ms_c,
+                          "radix_bits": plan.radix_bits, "tune_ms": tune,
                           "err_max": float(d.max()), "err_rms": float(torch.sqrt((d * d).mean()))}
             outs["f64"] = r.to(torch.int64)
             plan.close()
@@ -115,6 +124,12 @@ def main():
                                  stream=sid)
             out = torch.empty(batch * N, dtype=torch.int64, device=dev)
             err = torch.empty(2 * batch, dtype=torch.float64, device=dev)
+            tune = {}
+            for rbv in ([args.rb_f32] if args.rb_f32 else [3, 4, 5]):
+                plan.radix_bits = rbv
+                tune[rbv] = timed(lambda: plan.multiply_i32(a32, b32, out, batch, stream=sid),
+                                  stream, max(3, args.steps // 4), 1)
+            plan.radix_bits = min(tune, key=tune.get)
             ms = timed(lambda: plan.multiply_i32(a32, b32, out, batch, stream=sid), stream,
                        args.steps, args.warmup)
             ms_t = timed(lambda: plan.multiply_i32(a32, b32, out, batch, stream=sid,
@@ -125,6 +140,7 @@ def main():
             torch.cuda.synchronize()
             e = err.view(batch, 2)
             res["f32split"] = {"ms": ms, "ms_transform": ms_t, "ms_combine": ms_c,
+                               "radix_bits": plan.radix_bits, "tune_ms": tune,
                                "limbs": "a:%dx%d-bit" % (ka, wa),
                                "err_max": float(e[:, 0].max()),
                                "err_rms": float(torch.sqrt(e[:, 1].sum() / (batch * N * ka)))}

@@ -133,9 +133,6 @@ __device__ __forceinline__ void group_stages(cplx<T>* v, const PassArgs<T>& a, c
     for (int st = 0; st < RHO; st++) {
         const int i = mode_descending(MODE) ? (RHO - 1 - st) : st;
         const int bpos = g.blo + q + i;  // position bit of this stage
-        // keep the twiddle loads of a stage inside that stage (bounds the
-        // number of registers held by loads in flight)
-        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int outer = 0; outer < (mode_special(MODE) ? (1 << i) : (GS >> (i + 1))); outer++) {
             const int j_first = mode_special(MODE) ? outer : (outer << (i + 1));
@@ -145,6 +142,57 @@ __device__ __forceinline__ void group_stages(cplx<T>* v, const PassArgs<T>& a, c
             for (int inner = 0; inner < (mode_special(MODE) ? (GS >> (i + 1)) : (1 << i)); inner++) {
                 const int j0 = mode_special(MODE) ? ((inner << (i + 1)) | outer)
                                                   : ((outer << (i + 1)) | inner);
+                const int j1 = j0 | (1 << i);
+                cplx<T> U = v[j0], V = v[j1];
+                if constexpr (mode_is_ct(MODE)) {
+                    cplx<T> Vw = V * w;
+                    v[j0] = U + Vw;
+                    v[j1] = U - Vw;
+                } else {
+                    v[j0] = U + V;
+                    v[j1] = (U - V) * w;
+                }
+            }
+        }
+    }
+}
+
+// Standard-mode stages with derived twiddles.  Inside a step over bits
+// [q, q+RHO) every twiddle of the stage with in-group bit i is
+//     table[base + ((P >> (q+RHO)) << s) + t] = table[base + ((P >> (q+RHO)) << s)]
+//                                             * exp(2 pi i bitrev_s(t) / 2^(s+1)),
+// s = RHO-1-i, t < 2^s the group bits above i (the reference tables hold
+// bit-reversed powers, fft_cpu.cu:200-211 and 434-446, so an index t below
+// a multiple of 2^s only adds a root of order 2^(s+1)).  One table load per
+// stage; wc[2^s + t] holds the root constants (s < 5).  The values differ
+// from direct table reads by rounding only.
+template <typename T, int MODE, int RHO>
+__device__ __forceinline__ void group_stages_derived(cplx<T>* v, const PassArgs<T>& a,
+                                                     const TileGeo& g, int rbase, int col, int q,
+                                                     const cplx<T>* __restrict__ wc)
+{
+    static_assert(!mode_special(MODE), "derived twiddles are for the standard modes");
+    constexpr int GS = 1 << RHO;
+#pragma unroll
+    for (int st = 0; st < RHO; st++) {
+        const int i = mode_descending(MODE) ? (RHO - 1 - st) : st;
+        const int s = RHO - 1 - i;
+        const int bpos = g.blo + q + i;
+        const int p0 = g.pos(rbase, col);
+        const cplx<T> base = bldc<T>(g.tab, twiddle_index<T, MODE>(a.n, a.nega, bpos, p0));
+#pragma unroll
+        for (int t = 0; t < (1 << s); t++) {
+            cplx<T> w = base;
+            if (t == 1 && s == 1) {
+                // times exp(+-i pi/2), exact: forward chains use the roots,
+                // inverse chains (Gentleman-Sande) their conjugates
+                w = mode_is_ct(MODE) ? mk<T>(-base.y, base.x) : mk<T>(base.y, -base.x);
+            } else if (t > 0) {
+                w = base * wc[(1 << s) + t];
+            }
+#pragma unroll
+            for (int lo = 0; lo < (1 << i); lo++) {
+                const int j0 = (t << (i + 1)) | lo;
                 const int j1 = j0 | (1 << i);
                 cplx<T> U = v[j0], V = v[j1];
                 if constexpr (mode_is_ct(MODE)) {
@@ -176,20 +224,25 @@ __device__ __forceinline__ void group_of(int tid, int k, int logC, int q, int& r
 // One radix-2^RHO step over tile-row bits [q, q+RHO): every thread takes
 // EPT/2^RHO groups of 2^RHO elements from LDS, runs the RHO stages in
 // registers and writes them back.
-template <typename T, int MODE, int E, int RHO, int NT>
+template <typename T, int MODE, int E, int RHO, int NT, bool DRV = false>
 __device__ __forceinline__ void radix_step(cplx<T>* lds, const PassArgs<T>& a, const TileGeo& g,
-                                           int q, int tid)
+                                           int q, int tid, const cplx<T>* wc = nullptr)
 {
     constexpr int GS = 1 << RHO;
     constexpr int GPT = (1 << E) / GS;
-#pragma unroll
+    // with derived twiddles (fused kernels, tight on registers) the groups
+    // of a thread are processed one after the other
+#pragma unroll(DRV ? 1 : GPT)
     for (int k = 0; k < GPT; k++) {
         int rbase, col;
         group_of<RHO, NT>(tid, k, g.logC, q, rbase, col);
         cplx<T> v[GS];
 #pragma unroll
         for (int j = 0; j < GS; j++) v[j] = lds[swz(((rbase | (j << q)) << g.logC) | col)];
-        group_stages<T, MODE, RHO>(v, a, g, rbase, col, q);
+        if constexpr (DRV)
+            group_stages_derived<T, MODE, RHO>(v, a, g, rbase, col, q, wc);
+        else
+            group_stages<T, MODE, RHO>(v, a, g, rbase, col, q);
 #pragma unroll
         for (int j = 0; j < GS; j++) lds[swz(((rbase | (j << q)) << g.logC) | col)] = v[j];
     }
@@ -213,30 +266,32 @@ __host__ __device__ __forceinline__ void step_range(bool descending, int K, int 
 
 __host__ __device__ __forceinline__ int num_steps(int K, int E) { return (K + E - 1) / E; }
 
-template <typename T, int MODE, int E, int NT>
+template <typename T, int MODE, int E, int NT, bool DRV = false>
 __device__ __forceinline__ void radix_step_rt(cplx<T>* lds, const PassArgs<T>& a, const TileGeo& g,
-                                              int q, int rho, int tid)
+                                              int q, int rho, int tid, const cplx<T>* wc)
 {
     switch (rho) {
-        case 1: radix_step<T, MODE, E, 1, NT>(lds, a, g, q, tid); break;
-        case 2: if constexpr (E >= 2) radix_step<T, MODE, E, 2, NT>(lds, a, g, q, tid); break;
-        case 3: if constexpr (E >= 3) radix_step<T, MODE, E, 3, NT>(lds, a, g, q, tid); break;
-        case 4: if constexpr (E >= 4) radix_step<T, MODE, E, 4, NT>(lds, a, g, q, tid); break;
-        case 5: if constexpr (E >= 5) radix_step<T, MODE, E, 5, NT>(lds, a, g, q, tid); break;
+        case 1: radix_step<T, MODE, E, 1, NT, DRV>(lds, a, g, q, tid, wc); break;
+        case 2: if constexpr (E >= 2) radix_step<T, MODE, E, 2, NT, DRV>(lds, a, g, q, tid, wc); break;
+        case 3: if constexpr (E >= 3) radix_step<T, MODE, E, 3, NT, DRV>(lds, a, g, q, tid, wc); break;
+        case 4: if constexpr (E >= 4) radix_step<T, MODE, E, 4, NT, DRV>(lds, a, g, q, tid, wc); break;
+        case 5: if constexpr (E >= 5) radix_step<T, MODE, E, 5, NT, DRV>(lds, a, g, q, tid, wc); break;
         default: break;
     }
 }
 
-// Runs steps [s_begin, s_end) of the pass, each between two barriers.
-template <typename T, int MODE, int E, int NT>
+// Runs steps [s_begin, s_end) of the pass, each between two barriers; the
+// steps are radix-2^RB (RB <= E; a thread holds 2^E elements).
+template <typename T, int MODE, int E, int NT, bool DRV = false, int RB = E>
 __device__ __forceinline__ void run_steps(cplx<T>* lds, const PassArgs<T>& a, const TileGeo& g,
-                                          int tid, int s_begin, int s_end)
+                                          int tid, int s_begin, int s_end,
+                                          const cplx<T>* wc = nullptr)
 {
     for (int s = s_begin; s < s_end; s++) {
         int q, rho;
-        step_range(mode_descending(MODE), a.K, E, s, q, rho);
+        step_range(mode_descending(MODE), a.K, RB, s, q, rho);
         __syncthreads();
-        radix_step_rt<T, MODE, E, NT>(lds, a, g, q, rho, tid);
+        radix_step_rt<T, MODE, E, NT, DRV>(lds, a, g, q, rho, tid, wc);
     }
     __syncthreads();
 }

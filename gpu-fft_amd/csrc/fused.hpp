@@ -42,6 +42,8 @@ template <typename T> struct ProdArgs {
     const cplx<T>* itab;   // inverse root table
     const cplx<T>* twist;  // FFNT twist table (L entries)
     const cplx<T>* untwist;
+    const cplx<T>* wc;     // stage root constants wc[2^s + t] (32 entries)
+    const cplx<T>* wci;    // their conjugates, for the inverse table
     const void* a;
     const void* b;
     long long in_stride;   // input elements per product
@@ -66,10 +68,9 @@ __device__ __forceinline__ int limb_digit(long long v, int w, int i, int k)
     return (int)d;
 }
 
-// Folded, twisted input at position P (x: this product's operand, tw: twist).
+// Folded input at position P (x: this product's operand), not yet twisted.
 template <typename T, int PL>
-__device__ __forceinline__ cplx<T> prod_input(rsrc_t x, rsrc_t tw, int L, int w, int limb, int k,
-                                              int P)
+__device__ __forceinline__ cplx<T> prod_input(rsrc_t x, int L, int w, int limb, int k, int P)
 {
     cplx<T> z;
     if constexpr (PL == PL_FOLD) {
@@ -86,66 +87,92 @@ __device__ __forceinline__ cplx<T> prod_input(rsrc_t x, rsrc_t tw, int L, int w,
         const int v1 = (int)__builtin_amdgcn_raw_buffer_load_b32(x, (unsigned)(P + L) * 4u, 0, 0);
         z = mk<T>((T)limb_digit(v0, w, limb, k), (T)limb_digit(v1, w, limb, k));
     }
-    return z * bldc<T>(tw, P);
+    return z;
 }
 
-// Row q of the top SB forward stages (bits n-1 .. S) applied to zz[j]
-// = input at position j 2^S + c.  The twiddles do not depend on c.
+// Row q of the top SB forward stages (bits n-1 .. S) of zz[j] = input at
+// position j 2^S + c.  Only the entries whose processed bits equal q's
+// survive each stage, and their twiddle, tab[base + (q >> (i+1))], is
+// uniform, so stage i (bit S+i, top first) is z[j] += z[j + 2^i] * wq[i]
+// with wq[i] = +-tab[...] (sign: bit i of q), j < 2^i.
 template <typename T, int SB>
-__device__ __forceinline__ cplx<T> top_forward_row(cplx<T>* zz, const ProdArgs<T>& a, int q)
+__device__ __forceinline__ void top_row_twiddles(const ProdArgs<T>& a, int q, cplx<T>* wq)
 {
-    constexpr int Q = 1 << SB;
 #pragma unroll
-    for (int st = 0; st < SB; st++) {
-        const int i = SB - 1 - st;
+    for (int i = 0; i < SB; i++) {
         const int b = a.S + i;
         const int base = a.nega ? (1 << (a.n - 1 - b)) : 0;
-#pragma unroll
-        for (int j0 = 0; j0 < Q; j0++) {
-            if (j0 & (1 << i)) continue;
-            const int j1 = j0 | (1 << i);
-            const cplx<T> w = ldc(a.tab + base + (j0 >> (i + 1)));
-            cplx<T> U = zz[j0], V = zz[j1] * w;
-            zz[j0] = U + V;
-            zz[j1] = U - V;
-        }
+        cplx<T> w = a.tab[base + (q >> (i + 1))];
+        if ((q >> i) & 1) w = mk<T>(-w.x, -w.y);
+        wq[i] = w;
     }
-    cplx<T> r = zz[0];
+}
+
+template <typename T, int SB>
+__device__ __forceinline__ cplx<T> top_forward_row(cplx<T>* zz, const cplx<T>* wq)
+{
 #pragma unroll
-    for (int j = 1; j < Q; j++)
-        if (j == q) r = zz[j];
-    return r;
+    for (int i = SB - 1; i >= 0; i--) {
+#pragma unroll
+        for (int j = 0; j < (1 << i); j++) zz[j] = zz[j] + zz[j + (1 << i)] * wq[i];
+    }
+    return zz[0];
 }
 
 template <typename T, int S, int SB, int E, int PL>
 __device__ __forceinline__ void load_sub(cplx<T>* lds, const ProdArgs<T>& a, rsrc_t x, rsrc_t tw,
-                                         int w, int limb, int k, int q, int tid)
+                                         int w, int limb, int k, const cplx<T>* wq, int tid)
 {
     constexpr int NT = (1 << S) >> E;
     constexpr int Q = 1 << SB;
+    constexpr int U = (SB >= 2) ? 2 : 4;  // elements in flight per thread
     const int L = 1 << a.n;
-#pragma unroll 1
-    for (int i = 0; i < (1 << E); i++) {
-        const int c = tid + i * NT;
-        cplx<T> zz[Q];
+    // twist[j 2^S + c] = twist[c] * twist[j 2^S]: one per-lane table load per
+    // position, the Q high factors are uniform
+    cplx<T> thi[Q];
 #pragma unroll
-        for (int j = 0; j < Q; j++) zz[j] = prod_input<T, PL>(x, tw, L, w, limb, k, (j << S) | c);
-        lds[swz(c)] = top_forward_row<T, SB>(zz, a, q);
+    for (int j = 0; j < Q; j++) thi[j] = a.twist[j << S];
+#pragma unroll 1
+    for (int i0 = 0; i0 < (1 << E); i0 += U) {
+        cplx<T> zz[U][Q];
+        cplx<T> tlo[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int c = tid + (i0 + u) * NT;
+            tlo[u] = bldc<T>(tw, c);
+#pragma unroll
+            for (int j = 0; j < Q; j++) zz[u][j] = prod_input<T, PL>(x, L, w, limb, k, (j << S) | c);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int c = tid + (i0 + u) * NT;
+#pragma unroll
+            for (int j = 0; j < Q; j++) zz[u][j] = zz[u][j] * (j ? tlo[u] * thi[j] : tlo[u]);
+            lds[swz(c)] = top_forward_row<T, SB>(zz[u], wq);
+        }
     }
 }
 
-// Last forward step (bits [0, E)) of the thread's contiguous group, into v.
-template <typename T, int E>
+// Last forward step (bits [0, RB)) of the thread's 2^(E-RB) groups, into
+// v[k 2^RB + j] = position ((tid + k NT) << RB) | j.
+template <typename T, int E, int RB, int NT>
 __device__ __forceinline__ void last_forward_step(const cplx<T>* lds, const PassArgs<T>& pf,
-                                                  const TileGeo& g, int tid, cplx<T>* v)
+                                                  const TileGeo& g, int tid, cplx<T>* v,
+                                                  const cplx<T>* wc)
 {
-    const int rbase = tid << E;
 #pragma unroll
-    for (int j = 0; j < (1 << E); j++) v[j] = lds[swz(rbase | j)];
-    group_stages<T, MODE_STD_FWD, E>(v, pf, g, rbase, 0, 0);
+    for (int k = 0; k < (1 << (E - RB)); k++) {
+        const int rbase = (tid + k * NT) << RB;
+        cplx<T> w[1 << RB];
+#pragma unroll
+        for (int j = 0; j < (1 << RB); j++) w[j] = lds[swz(rbase | j)];
+        group_stages_derived<T, MODE_STD_FWD, RB>(w, pf, g, rbase, 0, 0, wc);
+#pragma unroll
+        for (int j = 0; j < (1 << RB); j++) v[(k << RB) + j] = w[j];
+    }
 }
 
-template <typename T, int S, int SB, int E, int PL>
+template <typename T, int S, int SB, int E, int PL, int RB>
 __global__ __launch_bounds__((1 << S) >> E) void product_sub_kernel(ProdArgs<T> a)
 {
     constexpr int TILE = 1 << S;
@@ -180,7 +207,7 @@ __global__ __launch_bounds__((1 << S) >> E) void product_sub_kernel(ProdArgs<T> 
     g.tab = make_rsrc(a.tab);
     TileGeo gi = g;
     gi.tab = make_rsrc(a.itab);
-    const int nst = num_steps(S, E);
+    const int nst = num_steps(S, RB);
     const int nprod = a.ka * a.kb;
     const long long L = 1LL << a.n;
 
@@ -193,6 +220,9 @@ __global__ __launch_bounds__((1 << S) >> E) void product_sub_kernel(ProdArgs<T> 
     const int KA = PL == PL_FOLD ? 1 : a.ka;
     const int KB = PL == PL_FOLD ? 1 : a.kb;
 
+    cplx<T> wq[SB > 0 ? SB : 1];
+    top_row_twiddles<T, SB>(a, q, wq);
+
     cplx<T> breg[EPT];
     for (int jb = 0; jb < KB; jb++) {
         // an opaque copy of tid per iteration keeps the address arithmetic of
@@ -200,24 +230,31 @@ __global__ __launch_bounds__((1 << S) >> E) void product_sub_kernel(ProdArgs<T> 
         int t = tid;
         asm volatile("" : "+v"(t));
         if (jb) __syncthreads();
-        load_sub<T, S, SB, E, PL>(lds, a, xb, tw, a.wb, jb, KB, q, t);
-        run_steps<T, MODE_STD_FWD, E, NT>(lds, pf, g, t, 0, nst - 1);
-        last_forward_step<T, E>(lds, pf, g, t, breg);
+        load_sub<T, S, SB, E, PL>(lds, a, xb, tw, a.wb, jb, KB, wq, t);
+        run_steps<T, MODE_STD_FWD, E, NT, true, RB>(lds, pf, g, t, 0, nst - 1, a.wc);
+        last_forward_step<T, E, RB, NT>(lds, pf, g, t, breg, a.wc);
         for (int ia = 0; ia < KA; ia++) {
             int u = tid;
             asm volatile("" : "+v"(u));
             __syncthreads();
-            load_sub<T, S, SB, E, PL>(lds, a, xa, tw, a.wa, ia, KA, q, u);
-            run_steps<T, MODE_STD_FWD, E, NT>(lds, pf, g, u, 0, nst - 1);
-            cplx<T> v[EPT];
-            last_forward_step<T, E>(lds, pf, g, u, v);
+            load_sub<T, S, SB, E, PL>(lds, a, xa, tw, a.wa, ia, KA, wq, u);
+            run_steps<T, MODE_STD_FWD, E, NT, true, RB>(lds, pf, g, u, 0, nst - 1, a.wc);
+            // last forward step, product, first inverse step: in registers,
+            // one radix-2^RB group at a time
 #pragma unroll
-            for (int j = 0; j < EPT; j++) v[j] = v[j] * breg[j];
-            const int rbase = u << E;
-            group_stages<T, MODE_STD_INV, E>(v, pinv, gi, rbase, 0, 0);
+            for (int k = 0; k < (1 << (E - RB)); k++) {
+                const int rbase = (u + k * NT) << RB;
+                cplx<T> v[1 << RB];
 #pragma unroll
-            for (int j = 0; j < EPT; j++) lds[swz(rbase | j)] = v[j];
-            run_steps<T, MODE_STD_INV, E, NT>(lds, pinv, gi, u, 1, nst);
+                for (int j = 0; j < (1 << RB); j++) v[j] = lds[swz(rbase | j)];
+                group_stages_derived<T, MODE_STD_FWD, RB>(v, pf, g, rbase, 0, 0, a.wc);
+#pragma unroll
+                for (int j = 0; j < (1 << RB); j++) v[j] = v[j] * breg[(k << RB) + j];
+                group_stages_derived<T, MODE_STD_INV, RB>(v, pinv, gi, rbase, 0, 0, a.wci);
+#pragma unroll
+                for (int j = 0; j < (1 << RB); j++) lds[swz(rbase | j)] = v[j];
+            }
+            run_steps<T, MODE_STD_INV, E, NT, true, RB>(lds, pinv, gi, u, 1, nst, a.wci);
             if constexpr (SB == 0 && PL == PL_FOLD) {
                 // whole product in this workgroup: scale, untwist, unfold
                 T* o = a.out + p * a.out_stride;

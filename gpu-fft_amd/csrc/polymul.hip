@@ -27,10 +27,13 @@ struct gpufft_polymul_plan_s {
     int max_batch;
     int wa, ka, wb, kb;
     int S, SB;       // sub-problem split
+    int rb;          // radix bits per LDS step of the product kernel
     void* root;      // forward FFNT root table (L entries)
     void* iroot;
     void* twist;
     void* untwist;
+    void* wc;        // stage root constants and their conjugates (32 each)
+    void* wci;
     void* scratch;
     size_t scratch_bytes;
 };
@@ -40,6 +43,7 @@ namespace {
 constexpr int F64_MAX_S = 13;  // 2^13 doubles complex = 128 KiB of LDS
 constexpr int F32_MAX_S = 14;  // 2^14 floats complex = 128 KiB of LDS
 constexpr int MAX_SB = 3;
+
 
 template <typename T> using ProdKernel = void (*)(ProdArgs<T>);
 
@@ -51,13 +55,13 @@ __global__ void zero_kernel(double* p, long long n)
     if (i < n) p[i] = 0.0;
 }
 
-template <typename T, int PL> ProdKernel<T> prod_kernel(int S, int SB)
+template <typename T, int PL, int RB> ProdKernel<T> prod_kernel_rb(int S, int SB)
 {
     // E = 4 (double) / 5 (float): 512 threads per 128 KiB tile
     constexpr int E = sizeof(T) == 8 ? 4 : 5;
     constexpr int SMAX = sizeof(T) == 8 ? F64_MAX_S : F32_MAX_S;
 #define GFA_PK(SS, BB) \
-    if (S == SS && SB == BB) return &product_sub_kernel<T, SS, BB, E, PL>;
+    if (S == SS && SB == BB) return &product_sub_kernel<T, SS, BB, E, PL, RB>;
     if (SB > 0) {
         GFA_PK(SMAX, 1)
         GFA_PK(SMAX, 2)
@@ -79,12 +83,29 @@ template <typename T, int PL> ProdKernel<T> prod_kernel(int S, int SB)
     return nullptr;
 }
 
+// radix bits per LDS step: 3 or 4 (double), 3..5 (float); chosen per plan
+template <typename T, int PL> ProdKernel<T> prod_kernel(int S, int SB, int rb)
+{
+    switch (rb) {
+        case 3: return prod_kernel_rb<T, PL, 3>(S, SB);
+        case 4: return prod_kernel_rb<T, PL, 4>(S, SB);
+        case 5:
+            if constexpr (sizeof(T) == 4) return prod_kernel_rb<T, PL, 5>(S, SB);
+            return nullptr;
+        default: return nullptr;
+    }
+}
+
 template <typename T> int make_tables(gpufft_polymul_plan_s* pl, hipStream_t st)
 {
     size_t bytes = sizeof(cplx<T>) << pl->n;
     void** dst[4] = {&pl->root, &pl->iroot, &pl->twist, &pl->untwist};
     for (void** d : dst) {
         hipError_t e = hipMalloc(d, bytes);
+        if (e != hipSuccess) return fail_hip(e, "gpufft_polymul_plan_create: hipMalloc");
+    }
+    for (void** d : {&pl->wc, &pl->wci}) {
+        hipError_t e = hipMalloc(d, sizeof(cplx<T>) * 32);
         if (e != hipSuccess) return fail_hip(e, "gpufft_polymul_plan_create: hipMalloc");
     }
     int (*gen)(void*, int, int, int, void*) =
@@ -94,13 +115,15 @@ template <typename T> int make_tables(gpufft_polymul_plan_s* pl, hipStream_t st)
     if ((s = gen(pl->iroot, GPUFFT_TABLE_FFNT_ROOT, pl->n, 1, st))) return s;
     if ((s = gen(pl->twist, GPUFFT_TABLE_FFNT_TWIST, pl->n, 0, st))) return s;
     if ((s = gen(pl->untwist, GPUFFT_TABLE_FFNT_TWIST, pl->n, 1, st))) return s;
+    if ((s = gen(pl->wc, GPUFFT_TABLE_STAGE_ROOTS, 5, 0, st))) return s;
+    if ((s = gen(pl->wci, GPUFFT_TABLE_STAGE_ROOTS, 5, 1, st))) return s;
     return GPUFFT_OK;
 }
 
 void release(gpufft_polymul_plan_s* pl)
 {
     if (!pl) return;
-    void* ptrs[5] = {pl->root, pl->iroot, pl->twist, pl->untwist, pl->scratch};
+    void* ptrs[7] = {pl->root, pl->iroot, pl->twist, pl->untwist, pl->wc, pl->wci, pl->scratch};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete pl;
@@ -119,6 +142,8 @@ int launch_products(const gpufft_polymul_plan_s* pl, const void* a, const void* 
     pa.itab = (const cplx<T>*)pl->iroot;
     pa.twist = (const cplx<T>*)pl->twist;
     pa.untwist = (const cplx<T>*)pl->untwist;
+    pa.wc = (const cplx<T>*)pl->wc;
+    pa.wci = (const cplx<T>*)pl->wci;
     pa.a = a;
     pa.b = b;
     pa.in_stride = 2LL << pl->n;
@@ -130,7 +155,7 @@ int launch_products(const gpufft_polymul_plan_s* pl, const void* a, const void* 
     pa.ka = pl->ka;
     pa.wb = pl->wb;
     pa.kb = pl->kb;
-    ProdKernel<T> k = prod_kernel<T, PL>(pl->S, pl->SB);
+    ProdKernel<T> k = prod_kernel<T, PL>(pl->S, pl->SB, pl->rb);
     if (!k) return fail(GPUFFT_ERR_UNSUPPORTED, std::string(who) + ": no kernel for this size");
     constexpr int E = sizeof(T) == 8 ? 4 : 5;
     const int threads = (1 << pl->S) >> E;
@@ -192,6 +217,7 @@ int gpufft_polymul_plan_create(gpufft_polymul_plan* out, int ring_power, int pre
     }
     pl->S = std::min(n, smax);
     pl->SB = n - pl->S;
+    pl->rb = 4;
     hipStream_t st = (hipStream_t)stream;
     int s = precision == GPUFFT_PREC_F64 ? make_tables<double>(pl, st) : make_tables<float>(pl, st);
     if (s) {
@@ -331,6 +357,22 @@ int gpufft_polymul_i32_phases(gpufft_polymul_plan plan, const int32_t* a, const 
 size_t gpufft_polymul_plan_scratch_bytes(gpufft_polymul_plan plan)
 {
     return plan ? plan->scratch_bytes : 0;
+}
+
+int gpufft_polymul_plan_set_radix_bits(gpufft_polymul_plan plan, int radix_bits)
+{
+    const char* who = "gpufft_polymul_plan_set_radix_bits";
+    if (!plan) return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": plan is NULL");
+    const int hi = plan->precision == GPUFFT_PREC_F64 ? 4 : 5;
+    if (radix_bits < 3 || radix_bits > hi)
+        return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": radix_bits out of range");
+    plan->rb = radix_bits;
+    return ok();
+}
+
+int gpufft_polymul_plan_get_radix_bits(gpufft_polymul_plan plan)
+{
+    return plan ? plan->rb : -1;
 }
 
 }  // extern "C"

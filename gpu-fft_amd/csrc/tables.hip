@@ -49,6 +49,17 @@ __global__ void table_kernel(cplx<T>* out, int kind, int count, int logm, int in
             num = k;
             den = 4LL * count;
             break;
+        case GPUFFT_TABLE_STAGE_ROOTS: {  // [2^s + t] = exp(2 pi i bitrev_s(t) / 2^(s+1))
+            if (k == 0) {
+                num = 0;
+                den = 1;
+            } else {
+                const int s = 31 - __clz(k);
+                num = brev(k - (1 << s), s);
+                den = 2LL << s;
+            }
+            break;
+        }
         default: {  // GPUFFT_TABLE_SPECIAL: T[h+j] = zeta_{8h}^{5^j mod 8h}
             if (k == 0) {
                 num = 0;
@@ -72,7 +83,7 @@ template <typename T> int gen(void* dst, int kind, int n_power, int inverse, voi
     if (!dst) return fail(GPUFFT_ERR_NULL_POINTER, "gpufft_generate_table: dst is NULL");
     if (n_power < 0 || n_power > 26)
         return fail(GPUFFT_ERR_INVALID_ARGUMENT, "gpufft_generate_table: n_power out of range");
-    if (kind < GPUFFT_TABLE_FFT || kind > GPUFFT_TABLE_SPECIAL)
+    if (kind < GPUFFT_TABLE_FFT || kind > GPUFFT_TABLE_STAGE_ROOTS)
         return fail(GPUFFT_ERR_INVALID_ARGUMENT, "gpufft_generate_table: unknown kind");
     const int count = 1 << n_power;
     hipLaunchKernelGGL(table_kernel<T>, dim3((count + 255) / 256), dim3(256), 0,

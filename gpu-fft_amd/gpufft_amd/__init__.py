@@ -76,6 +76,8 @@ _SIGS = {
     "gpufft_polymul_f64": [_P, _P, _P, _P, _I, _P],
     "gpufft_polymul_i32": [_P, _P, _P, _P, _I, _P, _P],
     "gpufft_polymul_f64_phases": [_P, _P, _P, _P, _I, _I, _P],
+    "gpufft_polymul_plan_set_radix_bits": [_P, _I],
+    "gpufft_polymul_plan_get_radix_bits": [_P],
     "gpufft_polymul_i32_phases": [_P, _P, _P, _P, _I, _P, _I, _P],
 }
 
@@ -213,6 +215,14 @@ class PolymulPlan:
         self.handle = h
         self.ring_power = ring_power
         self.precision = precision
+
+    @property
+    def radix_bits(self):
+        return _lib.gpufft_polymul_plan_get_radix_bits(self.handle)
+
+    @radix_bits.setter
+    def radix_bits(self, rb):
+        check(_lib.gpufft_polymul_plan_set_radix_bits(self.handle, rb))
 
     @property
     def scratch_bytes(self):

@@ -116,12 +116,15 @@ int gpufft_special_fft_f64(void* inout, const void* root_table, const gpufft_con
  *   GPUFFT_TABLE_SPECIAL     special-FFT table of 2^n_power slots,
  *                            T[h+j] = exp(2 pi i (5^j mod 8h)/8h) (layout
  *                            indexed at fft.cu:782,908); inverse: conjugate
+ *   GPUFFT_TABLE_STAGE_ROOTS [2^s + t] = exp(2 pi i bitrev_s(t) / 2^(s+1)),
+ *                            the in-step root constants of the fused kernels
  * The reference builds these on the host with libm (same layout and values
  * to within an ulp); this entry point has no counterpart there. */
 #define GPUFFT_TABLE_FFT 0
 #define GPUFFT_TABLE_FFNT_ROOT 1
 #define GPUFFT_TABLE_FFNT_TWIST 2
 #define GPUFFT_TABLE_SPECIAL 3
+#define GPUFFT_TABLE_STAGE_ROOTS 4
 int gpufft_generate_table_f64(void* dst, int kind, int n_power, int inverse, void* stream);
 int gpufft_generate_table_f32(void* dst, int kind, int n_power, int inverse, void* stream);
 
@@ -155,6 +158,12 @@ int gpufft_polymul_plan_create(gpufft_polymul_plan* out, int ring_power, int pre
                                int b_limbs, void* stream);
 int gpufft_polymul_plan_destroy(gpufft_polymul_plan plan);
 size_t gpufft_polymul_plan_scratch_bytes(gpufft_polymul_plan plan);
+/* Radix of the LDS steps of the product kernel, 2^radix_bits: 3 or 4 for
+ * GPUFFT_PREC_F64, 3..5 for GPUFFT_PREC_F32_SPLIT (default 4).  Results do
+ * not depend on it beyond rounding; the autotuner (gpufft_amd.autotune)
+ * sets it per ring size and device. */
+int gpufft_polymul_plan_set_radix_bits(gpufft_polymul_plan plan, int radix_bits);
+int gpufft_polymul_plan_get_radix_bits(gpufft_polymul_plan plan);
 int gpufft_polymul_f64(gpufft_polymul_plan plan, const double* a, const double* b, double* out,
                        int batch, void* stream);
 int gpufft_polymul_i32(gpufft_polymul_plan plan, const int32_t* a, const int32_t* b, int64_t* out,

@@ -146,9 +146,11 @@ void orc_special_table_f64(int l, int inverse, double *tab)
         long long mod = 8LL * h;
         long long p5 = 1;
         for (int j = 0; j < h; j++) {
-            /* angle / pi = 2 r / (8h), exact in double */
-            double x = (double)(2 * p5) / (double)mod;
-            double c = cos(M_PI * x), s = sin(M_PI * x);
+            /* angle = 2 pi r / (8h), evaluated in long double and rounded
+             * once to double */
+            long double ang = 2.0L * 3.141592653589793238462643383279502884L * (long double)p5 /
+                              (long double)mod;
+            double c = (double)cosl(ang), s = (double)sinl(ang);
             tab[2 * (h + j)] = c;
             tab[2 * (h + j) + 1] = inverse ? -s : s;
             p5 = (p5 * 5) % mod;

@@ -48,14 +48,16 @@ def test_gpu_tables_match_reference_layout(n_power):
 
 
 # ---- FP64 fused product ----
+@pytest.mark.parametrize("rb", [3, 4])
 @pytest.mark.parametrize("ring_power", [7, 8, 12, 14, 15, 16, 17])
-def test_polymul_f64_exact(ring_power):
+def test_polymul_f64_exact(ring_power, rb):
     G = _g()
     import torch
     N = 1 << ring_power
     batch = 11  # not a multiple of the 8-XCD block grouping
     a, b = _operands(N, batch, SEED_BASE + 3 + ring_power)
     plan = G.PolymulPlan(ring_power, G.PREC_F64, max_batch=batch)
+    plan.radix_bits = rb
     out = torch.empty(batch * N, dtype=torch.float64, device="cuda")
     plan.multiply_f64(to_dev(a.astype(np.float64).reshape(-1)), to_dev(b.astype(np.float64).reshape(-1)),
                       out, batch)
@@ -99,9 +101,10 @@ SPLITS = {
 }
 
 
+@pytest.mark.parametrize("rb", [3, 4, 5])
 @pytest.mark.parametrize("split", sorted(SPLITS))
 @pytest.mark.parametrize("ring_power", [8, 12, 15, 16])
-def test_polymul_f32_split_exact(ring_power, split):
+def test_polymul_f32_split_exact(ring_power, split, rb):
     G = _g()
     import torch
     N = 1 << ring_power
@@ -109,6 +112,7 @@ def test_polymul_f32_split_exact(ring_power, split):
     a, b = _operands(N, batch, SEED_BASE + 33 + ring_power)
     wa, ka, wb, kb = SPLITS[split]
     plan = G.PolymulPlan(ring_power, G.PREC_F32_SPLIT, batch, wa, ka, wb, kb)
+    plan.radix_bits = rb
     out = torch.empty(batch * N, dtype=torch.int64, device="cuda")
     err = torch.empty(batch * 2, dtype=torch.float64, device="cuda")
     plan.multiply_i32(to_dev(a.astype(np.int32).reshape(-1)), to_dev(b.astype(np.int32).reshape(-1)),
