@@ -5,7 +5,10 @@
  * (Alisah-Ozcan/GPU-FFT) it restates.
  *
  * Built with -ffp-contract=off so that products and sums round exactly as
- * written, the way the reference's host code (g++, no FMA) rounds them.
+ * written, the way the reference's host code (g++, no FMA) rounds them, and
+ * with -fno-builtin so that cos(x) and sin(x) stay two libm calls (a C
+ * compiler may fuse them into sincos(), which differs from them in the last
+ * bit for some x; the reference's C++ host code calls std::cos and std::sin).
  */
 #include "oracle.h"
 
