@@ -296,6 +296,28 @@ __device__ __forceinline__ void run_steps(cplx<T>* lds, const PassArgs<T>& a, co
     __syncthreads();
 }
 
+// Steps [S, SEND) of a pass of K bits with radix-2^RB steps, unrolled at
+// compile time so that every step's bit range is a constant (address and
+// twiddle-index arithmetic then folds).
+template <typename T, int MODE, int E, int NT, int K, int RB, int S, int SEND, bool DRV>
+__device__ __forceinline__ void run_steps_ct(cplx<T>* lds, const PassArgs<T>& a, const TileGeo& g,
+                                             int tid, const cplx<T>* wc)
+{
+    if constexpr (S < SEND) {
+        constexpr int full = K / RB;
+        constexpr int rem = K - full * RB;
+        constexpr int nst = full + (rem ? 1 : 0);
+        constexpr int as = mode_descending(MODE) ? (nst - 1 - S) : S;
+        constexpr int q = as * RB;
+        constexpr int rho = (as < full) ? RB : rem;
+        __syncthreads();
+        radix_step<T, MODE, E, rho, NT, DRV>(lds, a, g, q, tid, wc);
+        run_steps_ct<T, MODE, E, NT, K, RB, S + 1, SEND, DRV>(lds, a, g, tid, wc);
+    } else {
+        __syncthreads();
+    }
+}
+
 template <typename T, int MODE, int LOGT, int E, int LK, int SK>
 __global__ __launch_bounds__((1 << LOGT) >> E) void pass_kernel(PassArgs<T> a)
 {

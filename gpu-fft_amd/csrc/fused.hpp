@@ -52,42 +52,41 @@ template <typename T> struct ProdArgs {
     long long out_stride;
     cplx<T> scale;
     int wa, ka, wb, kb;    // limb split of a and b (PL_FOLD_LIMB)
+    long long oa, ob;      // digit offsets: sum_{k < K-1} 2^(w-1) 2^(w k)
 };
 
-// Balanced base-2^w digit i of v; the top digit (i == k-1) keeps the rest.
-__device__ __forceinline__ int limb_digit(long long v, int w, int i, int k)
+// Balanced base-2^w digit i of v (digits d_0 .. d_{k-1}, the top one keeps
+// the rest): with o = sum_{j < k-1} 2^(w-1) 2^(w j), the unsigned base-2^w
+// digits u_j of v + o give d_j = u_j - 2^(w-1) for j < k-1, and the top digit
+// is (v + o) >> (w (k-1)) (floor).
+__device__ __forceinline__ int limb_digit(int v, int w, int i, int k, long long o)
 {
-    for (int t = 0; t < i; t++) {
-        long long d = v & ((1LL << w) - 1);
-        if (d >= (1LL << (w - 1))) d -= (1LL << w);
-        v = (v - d) >> w;
-    }
-    if (i == k - 1) return (int)v;
-    long long d = v & ((1LL << w) - 1);
-    if (d >= (1LL << (w - 1))) d -= (1LL << w);
-    return (int)d;
+    if (k == 1) return v;
+    const long long u = (long long)v + o;
+    if (i == k - 1) return (int)(u >> (w * (k - 1)));
+    return (int)((u >> (w * i)) & ((1LL << w) - 1)) - (1 << (w - 1));
 }
 
-// Folded input at position P (x: this product's operand), not yet twisted.
+// Folded input (x_P, x_{P+L}) of this product's operand, as T.
 template <typename T, int PL>
-__device__ __forceinline__ cplx<T> prod_input(rsrc_t x, int L, int w, int limb, int k, int P)
+__device__ __forceinline__ void prod_input(rsrc_t x, int L, int w, int limb, int k, long long o,
+                                           int P, T& re, T& im)
 {
-    cplx<T> z;
     if constexpr (PL == PL_FOLD) {
         const unsigned o0 = (unsigned)P * sizeof(T), o1 = (unsigned)(P + L) * sizeof(T);
         if constexpr (sizeof(T) == 8) {
-            z.x = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(x, o0, 0, 0));
-            z.y = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(x, o1, 0, 0));
+            re = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(x, o0, 0, 0));
+            im = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(x, o1, 0, 0));
         } else {
-            z.x = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(x, o0, 0, 0));
-            z.y = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(x, o1, 0, 0));
+            re = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(x, o0, 0, 0));
+            im = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(x, o1, 0, 0));
         }
     } else {
         const int v0 = (int)__builtin_amdgcn_raw_buffer_load_b32(x, (unsigned)P * 4u, 0, 0);
         const int v1 = (int)__builtin_amdgcn_raw_buffer_load_b32(x, (unsigned)(P + L) * 4u, 0, 0);
-        z = mk<T>((T)limb_digit(v0, w, limb, k), (T)limb_digit(v1, w, limb, k));
+        re = (T)limb_digit(v0, w, limb, k, o);
+        im = (T)limb_digit(v1, w, limb, k, o);
     }
-    return z;
 }
 
 // Row q of the top SB forward stages (bits n-1 .. S) of zz[j] = input at
@@ -119,36 +118,57 @@ __device__ __forceinline__ cplx<T> top_forward_row(cplx<T>* zz, const cplx<T>* w
     return zz[0];
 }
 
+// Row q of the top network is linear in the inputs and the twist factors
+// as twist[j 2^S + c] = twist[c] * twist[j 2^S] (up to rounding), so
+//     y_q[c] = twist[c] * sum_j C[j] z_j,  C[j] = row_q(e_j) * twist[j 2^S],
+// with uniform coefficients C computed once per workgroup.
+template <typename T, int SB>
+__device__ __forceinline__ void top_row_coeffs(const ProdArgs<T>& a, int q, cplx<T>* C)
+{
+    constexpr int Q = 1 << SB;
+    cplx<T> wq[SB > 0 ? SB : 1];
+    top_row_twiddles<T, SB>(a, q, wq);
+#pragma unroll
+    for (int j = 0; j < Q; j++) {
+        cplx<T> e[Q];
+#pragma unroll
+        for (int k = 0; k < Q; k++) e[k] = mk<T>(T(0), T(0));
+        e[j] = a.twist[j << a.S];
+        C[j] = top_forward_row<T, SB>(e, wq);
+    }
+}
+
 template <typename T, int S, int SB, int E, int PL>
 __device__ __forceinline__ void load_sub(cplx<T>* lds, const ProdArgs<T>& a, rsrc_t x, rsrc_t tw,
-                                         int w, int limb, int k, const cplx<T>* wq, int tid)
+                                         int w, int limb, int k, long long o, const cplx<T>* C,
+                                         int tid)
 {
     constexpr int NT = (1 << S) >> E;
     constexpr int Q = 1 << SB;
     constexpr int U = (SB >= 2) ? 2 : 4;  // elements in flight per thread
     const int L = 1 << a.n;
-    // twist[j 2^S + c] = twist[c] * twist[j 2^S]: one per-lane table load per
-    // position, the Q high factors are uniform
-    cplx<T> thi[Q];
-#pragma unroll
-    for (int j = 0; j < Q; j++) thi[j] = a.twist[j << S];
 #pragma unroll 1
     for (int i0 = 0; i0 < (1 << E); i0 += U) {
-        cplx<T> zz[U][Q];
+        T re[U][Q], im[U][Q];
         cplx<T> tlo[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int c = tid + (i0 + u) * NT;
             tlo[u] = bldc<T>(tw, c);
 #pragma unroll
-            for (int j = 0; j < Q; j++) zz[u][j] = prod_input<T, PL>(x, L, w, limb, k, (j << S) | c);
+            for (int j = 0; j < Q; j++)
+                prod_input<T, PL>(x, L, w, limb, k, o, (j << S) | c, re[u][j], im[u][j]);
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int c = tid + (i0 + u) * NT;
+            T ax = T(0), ay = T(0);
 #pragma unroll
-            for (int j = 0; j < Q; j++) zz[u][j] = zz[u][j] * (j ? tlo[u] * thi[j] : tlo[u]);
-            lds[swz(c)] = top_forward_row<T, SB>(zz[u], wq);
+            for (int j = 0; j < Q; j++) {
+                ax = ax + C[j].x * re[u][j] - C[j].y * im[u][j];
+                ay = ay + C[j].x * im[u][j] + C[j].y * re[u][j];
+            }
+            lds[swz(c)] = tlo[u] * mk<T>(ax, ay);
         }
     }
 }
@@ -207,7 +227,7 @@ __global__ __launch_bounds__((1 << S) >> E) void product_sub_kernel(ProdArgs<T> 
     g.tab = make_rsrc(a.tab);
     TileGeo gi = g;
     gi.tab = make_rsrc(a.itab);
-    const int nst = num_steps(S, RB);
+    constexpr int NST = (S + RB - 1) / RB;
     const int nprod = a.ka * a.kb;
     const long long L = 1LL << a.n;
 
@@ -220,8 +240,8 @@ __global__ __launch_bounds__((1 << S) >> E) void product_sub_kernel(ProdArgs<T> 
     const int KA = PL == PL_FOLD ? 1 : a.ka;
     const int KB = PL == PL_FOLD ? 1 : a.kb;
 
-    cplx<T> wq[SB > 0 ? SB : 1];
-    top_row_twiddles<T, SB>(a, q, wq);
+    cplx<T> C[Q];
+    top_row_coeffs<T, SB>(a, q, C);
 
     cplx<T> breg[EPT];
     for (int jb = 0; jb < KB; jb++) {
@@ -230,15 +250,15 @@ __global__ __launch_bounds__((1 << S) >> E) void product_sub_kernel(ProdArgs<T> 
         int t = tid;
         asm volatile("" : "+v"(t));
         if (jb) __syncthreads();
-        load_sub<T, S, SB, E, PL>(lds, a, xb, tw, a.wb, jb, KB, wq, t);
-        run_steps<T, MODE_STD_FWD, E, NT, true, RB>(lds, pf, g, t, 0, nst - 1, a.wc);
+        load_sub<T, S, SB, E, PL>(lds, a, xb, tw, a.wb, jb, KB, a.ob, C, t);
+        run_steps_ct<T, MODE_STD_FWD, E, NT, S, RB, 0, NST - 1, true>(lds, pf, g, t, a.wc);
         last_forward_step<T, E, RB, NT>(lds, pf, g, t, breg, a.wc);
         for (int ia = 0; ia < KA; ia++) {
             int u = tid;
             asm volatile("" : "+v"(u));
             __syncthreads();
-            load_sub<T, S, SB, E, PL>(lds, a, xa, tw, a.wa, ia, KA, wq, u);
-            run_steps<T, MODE_STD_FWD, E, NT, true, RB>(lds, pf, g, u, 0, nst - 1, a.wc);
+            load_sub<T, S, SB, E, PL>(lds, a, xa, tw, a.wa, ia, KA, a.oa, C, u);
+            run_steps_ct<T, MODE_STD_FWD, E, NT, S, RB, 0, NST - 1, true>(lds, pf, g, u, a.wc);
             // last forward step, product, first inverse step: in registers,
             // one radix-2^RB group at a time
 #pragma unroll
@@ -254,7 +274,7 @@ __global__ __launch_bounds__((1 << S) >> E) void product_sub_kernel(ProdArgs<T> 
 #pragma unroll
                 for (int j = 0; j < (1 << RB); j++) lds[swz(rbase | j)] = v[j];
             }
-            run_steps<T, MODE_STD_INV, E, NT, true, RB>(lds, pinv, gi, u, 1, nst, a.wci);
+            run_steps_ct<T, MODE_STD_INV, E, NT, S, RB, 1, NST, true>(lds, pinv, gi, u, a.wci);
             if constexpr (SB == 0 && PL == PL_FOLD) {
                 // whole product in this workgroup: scale, untwist, unfold
                 T* o = a.out + p * a.out_stride;

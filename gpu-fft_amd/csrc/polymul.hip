@@ -155,6 +155,13 @@ int launch_products(const gpufft_polymul_plan_s* pl, const void* a, const void* 
     pa.ka = pl->ka;
     pa.wb = pl->wb;
     pa.kb = pl->kb;
+    auto offset = [](int w, int k) {
+        long long o = 0;
+        for (int j = 0; j + 1 < k; j++) o += (1LL << (w - 1)) << (w * j);
+        return o;
+    };
+    pa.oa = pl->ka > 1 ? offset(pl->wa, pl->ka) : 0;
+    pa.ob = pl->kb > 1 ? offset(pl->wb, pl->kb) : 0;
     ProdKernel<T> k = prod_kernel<T, PL>(pl->S, pl->SB, pl->rb);
     if (!k) return fail(GPUFFT_ERR_UNSUPPORTED, std::string(who) + ": no kernel for this size");
     constexpr int E = sizeof(T) == 8 ? 4 : 5;
