@@ -35,10 +35,18 @@ int log2_int(int n)
     return l;
 }
 
-// exp(i * theta) as the reference evaluates it: exp(0) * cos, exp(0) * sin
+// exp(i * theta) as the reference's Release build evaluates it: its g++ -O3
+// host compile turns std::cos(x), std::sin(x) into one glibc sincos(x)
+// (CMakeLists.txt:31-33), which can differ from separate calls in the last
+// bit; call it explicitly.
+inline void sin_cos(double x, double* s, double* c) { ::sincos(x, s, c); }
+inline void sin_cos(float x, float* s, float* c) { ::sincosf(x, s, c); }
+
 template <typename T> COMPLEX<T> unit(T theta)
 {
-    return COMPLEX<T>(std::cos(theta), std::sin(theta));
+    T s, c;
+    sin_cos(theta, &s, &c);
+    return COMPLEX<T>(c, s);
 }
 
 // Cooley-Tukey over a length-2^logm vector, natural in, bit-reversed out;

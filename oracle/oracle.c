@@ -5,11 +5,13 @@
  * (Alisah-Ozcan/GPU-FFT) it restates.
  *
  * Built with -ffp-contract=off so that products and sums round exactly as
- * written, the way the reference's host code (g++, no FMA) rounds them, and
- * with -fno-builtin so that cos(x) and sin(x) stay two libm calls (a C
- * compiler may fuse them into sincos(), which differs from them in the last
- * bit for some x; the reference's C++ host code calls std::cos and std::sin).
+ * written, the way the reference's host code (g++, no FMA) rounds them.
+ * cos and sin of one angle are evaluated with glibc's sincos(): the
+ * reference's Release build (g++ -O3 on its host code, CMakeLists.txt:31-33)
+ * fuses std::cos(x), std::sin(x) into that call, whose result differs from
+ * separate cos()/sin() in the last bit for some x.
  */
+#define _GNU_SOURCE
 #include "oracle.h"
 
 #include <math.h>
@@ -70,6 +72,9 @@ DEF_CPLX(float, f32)
  * (fft_cpu.cu:200-226).  root = (2)*(pi)/(2n) as complex ops, exact for
  * power-of-two n; element = exp((0,1)*(i,0)*(root,0)) = (cos th, sin th)
  * with th = i*root rounded in T (complex.cuh:305-311). */
+static inline void sc_f64(double x, double *s, double *c) { sincos(x, s, c); }
+static inline void sc_f32(float x, float *s, float *c) { sincosf(x, s, c); }
+
 #define DEF_FFT_TABLES(T, S, COS, SIN)                                              \
     void orc_fft_tables_##S(int n, T *root, T *inv, T *rev, T *inv_rev)             \
     {                                                                               \
@@ -80,8 +85,9 @@ DEF_CPLX(float, f32)
         c_##S *rt = (c_##S *)malloc(sizeof(c_##S) * (size_t)n);                     \
         c_##S *it = (c_##S *)malloc(sizeof(c_##S) * (size_t)n);                     \
         for (int i = 0; i < n; i++) {                                               \
-            T th = (T)i * r;                                                        \
-            rt[i] = mk_##S(COS(th), SIN(th));                                       \
+            T th = (T)i * r, sv, cv;                                                \
+            sc_##S(th, &sv, &cv);                                                   \
+            rt[i] = mk_##S(cv, sv);                                                 \
             it[i] = div_##S(one, rt[i]);                                            \
         }                                                                           \
         for (int i = 0; i < n; i++) {                                               \
@@ -113,11 +119,13 @@ DEF_FFT_TABLES(float, f32, cosf, sinf)
         c_##S *root = (c_##S *)malloc(sizeof(c_##S) * (size_t)L);                   \
         c_##S *iroot = (c_##S *)malloc(sizeof(c_##S) * (size_t)L);                  \
         for (int i = 0; i < L; i++) {                                               \
-            T th = (T)i * rn;                                                       \
-            root[i] = mk_##S(COS(th), SIN(th));                                     \
+            T th = (T)i * rn, sv, cv;                                               \
+            sc_##S(th, &sv, &cv);                                                   \
+            root[i] = mk_##S(cv, sv);                                               \
             iroot[i] = div_##S(one, root[i]);                                       \
             T tw = (T)i * rt;                                                       \
-            c_##S w = mk_##S(COS(tw), SIN(tw));                                     \
+            sc_##S(tw, &sv, &cv);                                                   \
+            c_##S w = mk_##S(cv, sv);                                               \
             c_##S uw = div_##S(one, w);                                             \
             if (twist) { twist[2 * i] = w.re; twist[2 * i + 1] = w.im; }            \
             if (untwist) { untwist[2 * i] = uw.re; untwist[2 * i + 1] = uw.im; }    \
@@ -237,11 +245,13 @@ void orc_cpu_ffnt_mult_f64(int N, const int64_t *a, const int64_t *b, int64_t *o
     c_f64 *x1 = (c_f64 *)malloc(sizeof(c_f64) * (size_t)L);
     c_f64 *x2 = (c_f64 *)malloc(sizeof(c_f64) * (size_t)L);
     for (int i = 0; i < L; i++) {
-        double th = (double)i * rn;
-        root[i] = mk_f64(cos(th), sin(th));
+        double th = (double)i * rn, sv, cv;
+        sincos(th, &sv, &cv);
+        root[i] = mk_f64(cv, sv);
         iroot[i] = div_f64(one, root[i]);
         double tt = (double)i * rtw;
-        tw[i] = mk_f64(cos(tt), sin(tt));
+        sincos(tt, &sv, &cv);
+        tw[i] = mk_f64(cv, sv);
         utw[i] = div_f64(one, tw[i]);
     }
     for (int i = 0; i < L; i++) {
