@@ -158,9 +158,54 @@ int special(void* inout, const void* table, const gpufft_config* cfg, int batch,
     return run_transform<T>(MODE_SPE_INV, n, batch, 0, (const cplx<T>*)table, io, st, who);
 }
 
+// Special FFT of double-complex slots in FP32 arithmetic: x = hi + lo with
+// hi = float(x), lo = float(x - hi); T(x) = T(hi) + T(lo) by linearity, each
+// transform in float, the two results added in double.  work: float complex
+// [2][batch][l] (T(hi), then the intermediate passes of T(lo)).
+int special_split(void* inout, const void* table, const gpufft_config* cfg, int batch, void* work,
+                  const char* who)
+{
+    if (int s = check_cfg(cfg, GPUFFT_MIN_N_POWER, GPUFFT_MAX_N_POWER, who)) return s;
+    if (int s = check_batch(batch, who)) return s;
+    if (!inout || !table || !work)
+        return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": NULL buffer");
+    const int n = cfg->n_power;
+    const long long L = 1LL << n;
+    hipStream_t st = (hipStream_t)cfg->stream;
+    const int mode = cfg->fft_type == GPUFFT_FORWARD ? MODE_SPE_FWD : MODE_SPE_INV;
+    cplx<float>* whi = (cplx<float>*)work;
+    cplx<float>* wlo = whi + (long long)batch * L;
+    TransformIO<float> io{};
+    io.src = inout;
+    io.src_stride = L;
+    io.do_scale = cfg->fft_type == GPUFFT_INVERSE;
+    io.scale = mk<float>((float)cfg->mod_inverse_re, (float)cfg->mod_inverse_im);
+    // T(hi) -> whi
+    io.lk = LD_SPLIT_HI;
+    io.sk = ST_CPLX;
+    io.dst = whi;
+    io.dst_stride = L;
+    io.work = whi;
+    if (int s = run_transform<float>(mode, n, batch, 0, (const cplx<float>*)table, io, st, who))
+        return s;
+    // T(lo), intermediates in wlo, last pass: inout = whi + T(lo) in double
+    io.lk = LD_SPLIT_LO;
+    io.sk = ST_SUM_F64;
+    io.dst = inout;
+    io.stab = whi;
+    io.work = wlo;
+    return run_transform<float>(mode, n, batch, 0, (const cplx<float>*)table, io, st, who);
+}
+
 }  // namespace
 
 extern "C" {
+
+int gpufft_special_fft_split_f32(void* inout, const void* root_table, const gpufft_config* cfg,
+                                 int batch, void* work)
+{
+    return special_split(inout, root_table, cfg, batch, work, "gpufft_special_fft_split_f32");
+}
 
 int gpufft_fft_c2c_f32(void* inout, const void* t, const gpufft_config* cfg, int batch, int mult)
 {

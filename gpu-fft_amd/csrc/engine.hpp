@@ -30,8 +30,20 @@
 namespace gfa {
 
 enum Mode { MODE_STD_FWD = 0, MODE_STD_INV = 1, MODE_SPE_FWD = 2, MODE_SPE_INV = 3 };
-enum LoadKind { LD_CPLX = 0, LD_REAL = 1, LD_MUL = 2, LD_FOLD = 3 };
-enum StoreKind { ST_CPLX = 0, ST_REAL = 1, ST_UNFOLD = 2 };
+enum LoadKind {
+    LD_CPLX = 0,
+    LD_REAL = 1,
+    LD_MUL = 2,
+    LD_FOLD = 3,
+    LD_SPLIT_HI = 4,  // T = float: double complex source, its float rounding
+    LD_SPLIT_LO = 5,  // T = float: double complex source, the float of the rest
+};
+enum StoreKind {
+    ST_CPLX = 0,
+    ST_REAL = 1,
+    ST_UNFOLD = 2,
+    ST_SUM_F64 = 3,  // T = float: double complex dst = (double)aux[p] + (double)z
+};
 
 __host__ __device__ constexpr bool mode_is_ct(int m) { return m == MODE_STD_FWD || m == MODE_SPE_FWD; }
 __host__ __device__ constexpr bool mode_descending(int m) { return m == MODE_STD_FWD || m == MODE_SPE_INV; }
@@ -55,7 +67,7 @@ template <typename T> struct PassArgs {
     long long src_stride;  // elements between transforms
     // output
     void* dst;
-    const cplx<T>* stab;  // untwist table (ST_UNFOLD)
+    const cplx<T>* stab;  // untwist table (ST_UNFOLD); first summand (ST_SUM_F64)
     long long dst_stride;
 };
 
@@ -74,10 +86,15 @@ __device__ __forceinline__ cplx<T> load_elem(const PassArgs<T>& a, long long b, 
         cplx<T> u = ldc(reinterpret_cast<const cplx<T>*>(a.src) + b * a.src_stride + p);
         cplx<T> v = ldc(reinterpret_cast<const cplx<T>*>(a.src2) + b * a.src_stride + p);
         return u * v;
-    } else {  // LD_FOLD: z = (f[p] + i f[p+L]) * twist[p]
+    } else if constexpr (LK == LD_FOLD) {  // z = (f[p] + i f[p+L]) * twist[p]
         const T* f = reinterpret_cast<const T*>(a.src) + b * a.src_stride;
         int L = 1 << a.n;
         return mk<T>(f[p], f[p + L]) * ldc(a.ltab + p);
+    } else {  // LD_SPLIT_HI / LD_SPLIT_LO: x = hi + lo, hi = float(x), lo = float(x - hi)
+        const cplx<double> x = ldc(reinterpret_cast<const cplx<double>*>(a.src) + b * a.src_stride + p);
+        const T hx = (T)x.x, hy = (T)x.y;
+        if constexpr (LK == LD_SPLIT_HI) return mk<T>(hx, hy);
+        return mk<T>((T)(x.x - (double)hx), (T)(x.y - (double)hy));
     }
 }
 
@@ -89,12 +106,16 @@ __device__ __forceinline__ void store_elem(const PassArgs<T>& a, long long b, in
         stc(reinterpret_cast<cplx<T>*>(a.dst) + b * a.dst_stride + p, z);
     } else if constexpr (SK == ST_REAL) {
         reinterpret_cast<T*>(a.dst)[b * a.dst_stride + p] = z.x;
-    } else {  // ST_UNFOLD: (z * untwist[p]) -> out[p] = re, out[p+L] = im
+    } else if constexpr (SK == ST_UNFOLD) {  // (z * untwist[p]) -> out[p] = re, out[p+L] = im
         z = z * ldc(a.stab + p);
         T* o = reinterpret_cast<T*>(a.dst) + b * a.dst_stride;
         int L = 1 << a.n;
         o[p] = z.x;
         o[p + L] = z.y;
+    } else {  // ST_SUM_F64
+        const cplx<T> h = ldc(a.stab + b * a.dst_stride + p);
+        stc(reinterpret_cast<cplx<double>*>(a.dst) + b * a.dst_stride + p,
+            mk<double>((double)h.x + (double)z.x, (double)h.y + (double)z.y));
     }
 }
 
@@ -313,6 +334,28 @@ __device__ __forceinline__ void run_steps_ct(cplx<T>* lds, const PassArgs<T>& a,
         __syncthreads();
         radix_step<T, MODE, E, rho, NT, DRV>(lds, a, g, q, tid, wc);
         run_steps_ct<T, MODE, E, NT, K, RB, S + 1, SEND, DRV>(lds, a, g, tid, wc);
+    } else {
+        __syncthreads();
+    }
+}
+
+// run_steps_ct for workgroups larger than the tile's thread count: every
+// thread takes the barriers, only the first NT threads compute.
+template <typename T, int MODE, int E, int NT, int K, int RB, int S, int SEND>
+__device__ __forceinline__ void run_steps_ct_active(cplx<T>* lds, const PassArgs<T>& a,
+                                                    const TileGeo& g, int tid, const cplx<T>* wc,
+                                                    bool active)
+{
+    if constexpr (S < SEND) {
+        constexpr int full = K / RB;
+        constexpr int rem = K - full * RB;
+        constexpr int nst = full + (rem ? 1 : 0);
+        constexpr int as = mode_descending(MODE) ? (nst - 1 - S) : S;
+        constexpr int q = as * RB;
+        constexpr int rho = (as < full) ? RB : rem;
+        __syncthreads();
+        if (active) radix_step<T, MODE, E, rho, NT, true>(lds, a, g, q, tid, wc);
+        run_steps_ct_active<T, MODE, E, NT, K, RB, S + 1, SEND>(lds, a, g, tid, wc, active);
     } else {
         __syncthreads();
     }

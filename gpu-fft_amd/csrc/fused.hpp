@@ -28,6 +28,10 @@
 
 namespace gfa {
 
+#ifndef GFA_LOAD_UNROLL
+#define GFA_LOAD_UNROLL 2
+#endif
+
 enum ProdLoad {
     PL_FOLD = 0,       // T inputs [batch][2^(n+1)]: fold + twist
     PL_FOLD_LIMB = 1,  // int32 inputs: balanced base-2^w digit, fold + twist
@@ -145,7 +149,7 @@ __device__ __forceinline__ void load_sub(cplx<T>* lds, const ProdArgs<T>& a, rsr
 {
     constexpr int NT = (1 << S) >> E;
     constexpr int Q = 1 << SB;
-    constexpr int U = (SB >= 2) ? 2 : 4;  // elements in flight per thread
+    constexpr int U = GFA_LOAD_UNROLL;  // elements in flight per thread
     const int L = 1 << a.n;
 #pragma unroll 1
     for (int i0 = 0; i0 < (1 << E); i0 += U) {

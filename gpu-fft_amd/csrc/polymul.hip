@@ -43,6 +43,9 @@ namespace {
 constexpr int F64_MAX_S = 13;  // 2^13 doubles complex = 128 KiB of LDS
 constexpr int F32_MAX_S = 14;  // 2^14 floats complex = 128 KiB of LDS
 constexpr int MAX_SB = 3;
+#ifndef GFA_E64
+#define GFA_E64 4
+#endif
 
 
 template <typename T> using ProdKernel = void (*)(ProdArgs<T>);
@@ -58,7 +61,7 @@ __global__ void zero_kernel(double* p, long long n)
 template <typename T, int PL, int RB> ProdKernel<T> prod_kernel_rb(int S, int SB)
 {
     // E = 4 (double) / 5 (float): 512 threads per 128 KiB tile
-    constexpr int E = sizeof(T) == 8 ? 4 : 5;
+    constexpr int E = sizeof(T) == 8 ? GFA_E64 : 5;
     constexpr int SMAX = sizeof(T) == 8 ? F64_MAX_S : F32_MAX_S;
 #define GFA_PK(SS, BB) \
     if (S == SS && SB == BB) return &product_sub_kernel<T, SS, BB, E, PL, RB>;
@@ -88,7 +91,9 @@ template <typename T, int PL> ProdKernel<T> prod_kernel(int S, int SB, int rb)
 {
     switch (rb) {
         case 3: return prod_kernel_rb<T, PL, 3>(S, SB);
-        case 4: return prod_kernel_rb<T, PL, 4>(S, SB);
+        case 4:
+            if constexpr (sizeof(T) == 4 || GFA_E64 >= 4) return prod_kernel_rb<T, PL, 4>(S, SB);
+            return nullptr;
         case 5:
             if constexpr (sizeof(T) == 4) return prod_kernel_rb<T, PL, 5>(S, SB);
             return nullptr;
@@ -164,7 +169,7 @@ int launch_products(const gpufft_polymul_plan_s* pl, const void* a, const void* 
     pa.ob = pl->kb > 1 ? offset(pl->wb, pl->kb) : 0;
     ProdKernel<T> k = prod_kernel<T, PL>(pl->S, pl->SB, pl->rb);
     if (!k) return fail(GPUFFT_ERR_UNSUPPORTED, std::string(who) + ": no kernel for this size");
-    constexpr int E = sizeof(T) == 8 ? 4 : 5;
+    constexpr int E = sizeof(T) == 8 ? GFA_E64 : 5;
     const int threads = (1 << pl->S) >> E;
     const long long groups = ((long long)batch + 7) / 8;
     const long long blocks = groups * 8 * (1LL << pl->SB);

@@ -99,6 +99,16 @@ template <typename T> PassFn<T> select_pass(int mode, int lk, int sk)
     GFA_SEL(MODE_STD_INV, LD_MUL, ST_UNFOLD)
     GFA_SEL(MODE_SPE_FWD, LD_CPLX, ST_CPLX)
     GFA_SEL(MODE_SPE_INV, LD_CPLX, ST_CPLX)
+    if constexpr (sizeof(T) == 4) {
+        GFA_SEL(MODE_SPE_FWD, LD_SPLIT_HI, ST_CPLX)
+        GFA_SEL(MODE_SPE_FWD, LD_SPLIT_LO, ST_CPLX)
+        GFA_SEL(MODE_SPE_FWD, LD_SPLIT_LO, ST_SUM_F64)
+        GFA_SEL(MODE_SPE_FWD, LD_CPLX, ST_SUM_F64)
+        GFA_SEL(MODE_SPE_INV, LD_SPLIT_HI, ST_CPLX)
+        GFA_SEL(MODE_SPE_INV, LD_SPLIT_LO, ST_CPLX)
+        GFA_SEL(MODE_SPE_INV, LD_SPLIT_LO, ST_SUM_F64)
+        GFA_SEL(MODE_SPE_INV, LD_CPLX, ST_SUM_F64)
+    }
 #undef GFA_SEL
     return nullptr;
 }

@@ -14,7 +14,10 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libgpufft_amd.so")
+# GPUFFT_AMD_LIB may name another build of the same library (e.g. one compiled
+# with different tuning macros); by default the in-tree build is loaded.
+LIB_PATH = os.environ.get("GPUFFT_AMD_LIB") or os.path.join(os.path.dirname(_HERE), "lib",
+                                                            "libgpufft_amd.so")
 
 FORWARD, INVERSE = 0, 1
 X_N_PLUS, X_N_MINUS = 0, 1
@@ -69,6 +72,7 @@ _SIGS = {
     "gpufft_ffnt_f64": [_P, _P, _P, _P, _CP, _I, _I],
     "gpufft_special_fft_f32": [_P, _P, _CP, _I],
     "gpufft_special_fft_f64": [_P, _P, _CP, _I],
+    "gpufft_special_fft_split_f32": [_P, _P, _CP, _I, _P],
     "gpufft_generate_table_f64": [_P, _I, _I, _I, _P],
     "gpufft_generate_table_f32": [_P, _I, _I, _I, _P],
     "gpufft_polymul_plan_create": [_P, _I, _I, _I, _I, _I, _I, _I, _P],
@@ -77,6 +81,12 @@ _SIGS = {
     "gpufft_polymul_i32": [_P, _P, _P, _P, _I, _P, _P],
     "gpufft_polymul_f64_phases": [_P, _P, _P, _P, _I, _I, _P],
     "gpufft_polymul_plan_set_radix_bits": [_P, _I],
+    "gpufft_batched_plan_create": [_P, _P, _I, _I, _I, _I, _I, _P],
+    "gpufft_batched_plan_destroy": [_P],
+    "gpufft_batched_plan_set_tuning": [_P, _I, _I],
+    "gpufft_batched_plan_info": [_P, _P, _P, _P, _P],
+    "gpufft_batched_execute": [_P, _P, _P, _P, _P, _P],
+    "gpufft_batched_graph_launch": [_P, _P, _P, _P, _P, _P],
     "gpufft_polymul_plan_get_radix_bits": [_P],
     "gpufft_polymul_i32_phases": [_P, _P, _P, _P, _I, _P, _I, _P],
 }
@@ -178,6 +188,12 @@ def GPU_FFNT(inout, temp, twist_table, root_table, cfg, batch_size, multiplicati
              batch_size, int(multiplication)))
 
 
+def special_fft_split(inout, root_table, cfg, batch_size, work):
+    """Special FFT of complex128 slots in FP32 arithmetic (hi/lo split)."""
+    check(_lib.gpufft_special_fft_split_f32(_ptr(inout), _ptr(root_table), ctypes.byref(cfg),
+                                            batch_size, _ptr(work)))
+
+
 def GPU_Special_FFT(inout, root_table, cfg, batch_size):
     """GPU_Special_FFT (reference fft.cuh:194-197)."""
     fn = getattr(_lib, "gpufft_special_fft_" + _prec(inout))
@@ -239,6 +255,81 @@ class PolymulPlan:
     def close(self):
         if self.handle:
             _lib.gpufft_polymul_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---- batched products of uneven ring sizes ----
+PRODUCT_NEGACYCLIC, PRODUCT_LINEAR = 0, 1
+
+
+class ProductDesc(ctypes.Structure):
+    """gpufft_product_desc (include/gpufft_amd.h)."""
+    _fields_ = [
+        ("kind", ctypes.c_int),
+        ("ring_power", ctypes.c_int),
+        ("a_offset", ctypes.c_longlong),
+        ("b_offset", ctypes.c_longlong),
+        ("out_offset", ctypes.c_longlong),
+    ]
+
+
+def output_length(kind, ring_power):
+    """int64 output elements of one product."""
+    return (1 << ring_power) * (1 if kind == PRODUCT_NEGACYCLIC else 2)
+
+
+def pack_products(kinds, ring_powers):
+    """Descriptors for products stored back to back: returns (descs, a/b
+    element count, output element count)."""
+    descs = (ProductDesc * len(kinds))()
+    io = oo = 0
+    for i, (k, r) in enumerate(zip(kinds, ring_powers)):
+        descs[i].kind, descs[i].ring_power = int(k), int(r)
+        descs[i].a_offset = descs[i].b_offset = io
+        descs[i].out_offset = oo
+        io += 1 << int(r)
+        oo += output_length(int(k), int(r))
+    return descs, io, oo
+
+
+class BatchedPlan:
+    """gpufft_batched_plan: uneven negacyclic / linear products, one launch."""
+
+    def __init__(self, descs, a_limb_bits=5, a_limbs=2, b_limb_bits=5, b_limbs=2, stream=None):
+        h = ctypes.c_void_p()
+        check(_lib.gpufft_batched_plan_create(ctypes.byref(h), ctypes.byref(descs), len(descs),
+                                              a_limb_bits, a_limbs, b_limb_bits, b_limbs,
+                                              _stream(stream)))
+        self.handle = h
+        self.count = len(descs)
+
+    def set_tuning(self, radix_bits, grid):
+        check(_lib.gpufft_batched_plan_set_tuning(self.handle, radix_bits, grid))
+
+    def info(self):
+        n, s, rb, g = ctypes.c_int(), ctypes.c_longlong(), ctypes.c_int(), ctypes.c_int()
+        check(_lib.gpufft_batched_plan_info(self.handle, ctypes.byref(n), ctypes.byref(s),
+                                            ctypes.byref(rb), ctypes.byref(g)))
+        return {"items": n.value, "scratch_elems": s.value, "radix_bits": rb.value,
+                "grid": g.value}
+
+    def execute(self, a, b, out, err_stats=None, stream=None):
+        check(_lib.gpufft_batched_execute(self.handle, _ptr(a), _ptr(b), _ptr(out),
+                                          _ptr(err_stats), _stream(stream)))
+
+    def graph_launch(self, a, b, out, err_stats=None, stream=None):
+        check(_lib.gpufft_batched_graph_launch(self.handle, _ptr(a), _ptr(b), _ptr(out),
+                                               _ptr(err_stats), _stream(stream)))
+
+    def close(self):
+        if self.handle:
+            _lib.gpufft_batched_plan_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -104,6 +104,16 @@ int gpufft_special_fft_f32(void* inout, const void* root_table, const gpufft_con
 int gpufft_special_fft_f64(void* inout, const void* root_table, const gpufft_config* cfg,
                            int batch);
 
+/* The same special FFT on double-complex slots, evaluated in FP32: each slot
+ * is split x = hi + lo (hi = float(x), lo = float(x - hi)), both halves are
+ * transformed with the float table (same layout) and the two results are
+ * added in double, so the 2^30-scaled CKKS slots are not truncated to float
+ * on input.  work: device buffer of 2 * batch * 2^n_power float complex.
+ * No counterpart in the reference (which runs GPU_Special_FFT<float> on float
+ * data); the FP64 entry point above is the drop-in. */
+int gpufft_special_fft_split_f32(void* inout, const void* root_table, const gpufft_config* cfg,
+                                 int batch, void* work);
+
 /* ---- root-of-unity tables generated on the GPU (FP64 arithmetic) ----
  * Fills dst[0 .. 2^n_power) on `stream` in the reference layouts:
  *   GPUFFT_TABLE_FFT         FFT<T>(2^n_power).ReverseRootTable()
@@ -183,6 +193,44 @@ int gpufft_polymul_f64_phases(gpufft_polymul_plan plan, const double* a, const d
 int gpufft_polymul_i32_phases(gpufft_polymul_plan plan, const int32_t* a, const int32_t* b,
                               int64_t* out, int batch, double* err_stats, int phases,
                               void* stream);
+
+/* ---- batched products of uneven ring sizes, one persistent launch ----
+ * (no counterpart in the reference, which needs one GPU_FFT/GPU_FFNT call
+ * pair per ring size).  Each product of the list is
+ *   GPUFFT_PRODUCT_NEGACYCLIC  out[0..N)  = a * b mod X^N + 1
+ *   GPUFFT_PRODUCT_LINEAR      out[0..2N) = a * b (2N-1 coefficients, then 0)
+ * with N = 2^ring_power coefficients per operand (NEGACYCLIC: ring_power
+ * 7..19, LINEAR: 5..17) at the given element offsets of the int32 inputs a,
+ * b and the int64 output.  Arithmetic is FP32 with the operands split into
+ * balanced limbs (as gpufft_polymul_i32) and exact int64 recombination.
+ * The plan (created once; it allocates its tables, work list and scratch)
+ * runs every product in one persistent kernel: gpufft_batched_execute
+ * enqueues it on `stream`; gpufft_batched_graph_launch captures it into a
+ * HIP graph on first use (per buffer set) and replays the graph.  err_stats
+ * (optional, double [count]) receives each product's largest distance of a
+ * limb output to its rounding (< 0.5: every limb was rounded exactly).
+ * Calls on one plan must not overlap. */
+#define GPUFFT_PRODUCT_NEGACYCLIC 0
+#define GPUFFT_PRODUCT_LINEAR 1
+typedef struct gpufft_product_desc {
+    int kind;
+    int ring_power;
+    long long a_offset, b_offset, out_offset;
+} gpufft_product_desc;
+typedef struct gpufft_batched_plan_s* gpufft_batched_plan;
+int gpufft_batched_plan_create(gpufft_batched_plan* out, const gpufft_product_desc* descs, int count,
+                               int a_limb_bits, int a_limbs, int b_limb_bits, int b_limbs,
+                               void* stream);
+int gpufft_batched_plan_destroy(gpufft_batched_plan plan);
+/* radix of the LDS steps (2^3..2^5) and persistent grid size (workgroups);
+ * chosen by the autotuner (gpufft_amd.autotune) */
+int gpufft_batched_plan_set_tuning(gpufft_batched_plan plan, int radix_bits, int grid);
+int gpufft_batched_plan_info(gpufft_batched_plan plan, int* n_items, long long* scratch_elems,
+                             int* radix_bits, int* grid);
+int gpufft_batched_execute(gpufft_batched_plan plan, const int32_t* a, const int32_t* b,
+                           int64_t* out, double* err_stats, void* stream);
+int gpufft_batched_graph_launch(gpufft_batched_plan plan, const int32_t* a, const int32_t* b,
+                                int64_t* out, double* err_stats, void* stream);
 
 /* ---- errors ---- */
 /* Human-readable description of the last failure on the calling thread. */
