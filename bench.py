@@ -5,7 +5,7 @@ b uniform in [-2^7, 2^7], seeded splitmix64), inputs already resident in HBM.
 Forms timed, all on one stream with HIP events:
   f64         fused FP64 product (double in / double out), gpufft_polymul_f64
   f32split    fused FP32 limb-split product (int32 in / int64 out),
-              gpufft_polymul_i32, a split into 5 balanced 5-bit limbs
+              gpufft_polymul_i32, a split into 6 balanced 4-bit limbs
   f64_dropin  the reference's two-call flow: GPU_FFNT FORWARD on 2*batch
               polynomials, then GPU_FFNT INVERSE with multiplication
 `value` is the poly-mults/s of the fastest form whose rounded products are
@@ -36,7 +36,7 @@ def parse():
     p.add_argument("--ring-power", type=int, default=16)
     p.add_argument("--batch", type=int, default=256)
     p.add_argument("--forms", default="f64,f32split,f64_dropin")
-    p.add_argument("--a-limbs", default="5x5", help="a limb split, bits x count")
+    p.add_argument("--a-limbs", default="4x6", help="a limb split, bits x count")
     p.add_argument("--cpu-seconds", type=float, default=10.0)
     p.add_argument("--rb-f64", type=int, default=0, help="radix bits of the FP64 kernel (0: tune)")
     p.add_argument("--rb-f32", type=int, default=0, help="radix bits of the FP32 kernel (0: tune)")

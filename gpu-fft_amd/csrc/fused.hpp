@@ -57,6 +57,7 @@ template <typename T> struct ProdArgs {
     cplx<T> scale;
     int wa, ka, wb, kb;    // limb split of a and b (PL_FOLD_LIMB)
     long long oa, ob;      // digit offsets: sum_{k < K-1} 2^(w-1) 2^(w k)
+    int* arrive;           // per-product arrival counters (in-kernel combine)
 };
 
 // Balanced base-2^w digit i of v (digits d_0 .. d_{k-1}, the top one keeps
@@ -196,7 +197,54 @@ __device__ __forceinline__ void last_forward_step(const cplx<T>* lds, const Pass
     }
 }
 
-template <typename T, int S, int SB, int E, int PL, int RB>
+// Top SB Gentleman-Sande stages (bits S .. n-1) of one whole product from
+// its 2^SB sub-problem results in scratch, then scale, untwist and unfold:
+// out[P] = Re, out[P + L] = Im (reference InverseFFNTCoreUnique,
+// src/lib/fft.cu:1298-1384).  Run by the last workgroup of the product.
+template <typename T, int S, int SB, int NT>
+__device__ __forceinline__ void combine_product(const ProdArgs<T>& a, long long p, int tid)
+{
+    constexpr int Q = 1 << SB;
+    const int L = 1 << a.n;
+    const rsrc_t src = make_rsrc(a.scratch + p * (long long)L);
+    const rsrc_t ut = make_rsrc(a.untwist);
+    T* o = a.out + p * a.out_stride;
+    // stage i, pair (j0, j0 + 2^i): twiddle itab[(j0 2^S + c) >> (S + i + 1)] = itab[j0 >> (i+1)]
+    cplx<T> w[Q / 2];
+#pragma unroll
+    for (int m = 0; m < Q / 2; m++) w[m] = a.itab[m];
+#pragma unroll 2
+    for (int c = tid; c < (1 << S); c += NT) {
+        cplx<T> y[Q];
+#pragma unroll
+        for (int j = 0; j < Q; j++) y[j] = bldc<T>(src, (j << S) | c);
+#pragma unroll
+        for (int i = 0; i < SB; i++) {
+#pragma unroll
+            for (int j0 = 0; j0 < Q; j0++) {
+                if (j0 & (1 << i)) continue;
+                const int j1 = j0 | (1 << i);
+                const cplx<T> U = y[j0], V = y[j1];
+                y[j0] = U + V;
+                y[j1] = (U - V) * w[j0 >> (i + 1)];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < Q; j++) {
+            const int P = (j << S) | c;
+            const cplx<T> z = (y[j] * a.scale) * bldc<T>(ut, P);
+            o[P] = z.x;
+            o[P + L] = z.y;
+        }
+    }
+}
+
+// CMB (FP64, SB > 0): the combine runs inside this kernel.  Each workgroup
+// publishes its sub-problem result (agent-scope release) and counts itself
+// in arrive[p]; the workgroup that arrives last acquires, combines the whole
+// product and resets arrive[p] to 0 for the next launch.  No workgroup waits
+// for another.
+template <typename T, int S, int SB, int E, int PL, int RB, bool CMB = false>
 __global__ __launch_bounds__((1 << S) >> E) void product_sub_kernel(ProdArgs<T> a)
 {
     constexpr int TILE = 1 << S;
@@ -298,6 +346,23 @@ __global__ __launch_bounds__((1 << S) >> E) void product_sub_kernel(ProdArgs<T> 
                     stc(dst + c, lds[swz(c)]);
                 }
             }
+        }
+    }
+    if constexpr (CMB && SB > 0 && PL == PL_FOLD) {
+        __shared__ int sh_last;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (tid == 0) {
+            const int prev = __hip_atomic_fetch_add(a.arrive + p, 1, __ATOMIC_ACQ_REL,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+            sh_last = prev == Q - 1;
+        }
+        __syncthreads();
+        if (sh_last) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            combine_product<T, S, SB, NT>(a, p, tid);
+            if (tid == 0)
+                __hip_atomic_store(a.arrive + p, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
     (void)Q;

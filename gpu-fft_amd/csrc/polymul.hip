@@ -28,6 +28,7 @@ struct gpufft_polymul_plan_s {
     int wa, ka, wb, kb;
     int S, SB;       // sub-problem split
     int rb;          // radix bits per LDS step of the product kernel
+    int cmb;         // FP64, SB > 0: 1 = top stages in the product kernel, 0 = separate pass (default)
     void* root;      // forward FFNT root table (L entries)
     void* iroot;
     void* twist;
@@ -36,6 +37,7 @@ struct gpufft_polymul_plan_s {
     void* wci;
     void* scratch;
     size_t scratch_bytes;
+    int* arrive;     // per-product arrival counters (max_batch), kept at 0 between launches
 };
 
 namespace {
@@ -58,11 +60,24 @@ __global__ void zero_kernel(double* p, long long n)
     if (i < n) p[i] = 0.0;
 }
 
-template <typename T, int PL, int RB> ProdKernel<T> prod_kernel_rb(int S, int SB)
+__global__ void zero_ints(int* p, int n)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+template <typename T, int PL, int RB> ProdKernel<T> prod_kernel_rb(int S, int SB, int cmb)
 {
     // E = 4 (double) / 5 (float): 512 threads per 128 KiB tile
     constexpr int E = sizeof(T) == 8 ? GFA_E64 : 5;
     constexpr int SMAX = sizeof(T) == 8 ? F64_MAX_S : F32_MAX_S;
+    if constexpr (sizeof(T) == 8 && PL == PL_FOLD) {
+        if (cmb && S == SMAX) {
+            if (SB == 1) return &product_sub_kernel<T, SMAX, 1, E, PL, RB, true>;
+            if (SB == 2) return &product_sub_kernel<T, SMAX, 2, E, PL, RB, true>;
+            if (SB == 3) return &product_sub_kernel<T, SMAX, 3, E, PL, RB, true>;
+        }
+    }
 #define GFA_PK(SS, BB) \
     if (S == SS && SB == BB) return &product_sub_kernel<T, SS, BB, E, PL, RB>;
     if (SB > 0) {
@@ -87,15 +102,15 @@ template <typename T, int PL, int RB> ProdKernel<T> prod_kernel_rb(int S, int SB
 }
 
 // radix bits per LDS step: 3 or 4 (double), 3..5 (float); chosen per plan
-template <typename T, int PL> ProdKernel<T> prod_kernel(int S, int SB, int rb)
+template <typename T, int PL> ProdKernel<T> prod_kernel(int S, int SB, int rb, int cmb)
 {
     switch (rb) {
-        case 3: return prod_kernel_rb<T, PL, 3>(S, SB);
+        case 3: return prod_kernel_rb<T, PL, 3>(S, SB, cmb);
         case 4:
-            if constexpr (sizeof(T) == 4 || GFA_E64 >= 4) return prod_kernel_rb<T, PL, 4>(S, SB);
+            if constexpr (sizeof(T) == 4 || GFA_E64 >= 4) return prod_kernel_rb<T, PL, 4>(S, SB, cmb);
             return nullptr;
         case 5:
-            if constexpr (sizeof(T) == 4) return prod_kernel_rb<T, PL, 5>(S, SB);
+            if constexpr (sizeof(T) == 4) return prod_kernel_rb<T, PL, 5>(S, SB, cmb);
             return nullptr;
         default: return nullptr;
     }
@@ -128,7 +143,8 @@ template <typename T> int make_tables(gpufft_polymul_plan_s* pl, hipStream_t st)
 void release(gpufft_polymul_plan_s* pl)
 {
     if (!pl) return;
-    void* ptrs[7] = {pl->root, pl->iroot, pl->twist, pl->untwist, pl->wc, pl->wci, pl->scratch};
+    void* ptrs[8] = {pl->root, pl->iroot, pl->twist, pl->untwist, pl->wc, pl->wci, pl->scratch,
+                     pl->arrive};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete pl;
@@ -136,7 +152,7 @@ void release(gpufft_polymul_plan_s* pl)
 
 template <typename T, int PL>
 int launch_products(const gpufft_polymul_plan_s* pl, const void* a, const void* b, T* out_direct,
-                    int batch, hipStream_t st, const char* who)
+                    int batch, int cmb, hipStream_t st, const char* who)
 {
     ProdArgs<T> pa{};
     pa.n = pl->n;
@@ -167,7 +183,8 @@ int launch_products(const gpufft_polymul_plan_s* pl, const void* a, const void* 
     };
     pa.oa = pl->ka > 1 ? offset(pl->wa, pl->ka) : 0;
     pa.ob = pl->kb > 1 ? offset(pl->wb, pl->kb) : 0;
-    ProdKernel<T> k = prod_kernel<T, PL>(pl->S, pl->SB, pl->rb);
+    pa.arrive = pl->arrive;
+    ProdKernel<T> k = prod_kernel<T, PL>(pl->S, pl->SB, pl->rb, cmb);
     if (!k) return fail(GPUFFT_ERR_UNSUPPORTED, std::string(who) + ": no kernel for this size");
     constexpr int E = sizeof(T) == 8 ? GFA_E64 : 5;
     const int threads = (1 << pl->S) >> E;
@@ -230,6 +247,7 @@ int gpufft_polymul_plan_create(gpufft_polymul_plan* out, int ring_power, int pre
     pl->S = std::min(n, smax);
     pl->SB = n - pl->S;
     pl->rb = 4;
+    pl->cmb = 0;  // separate pass: measured faster at N = 2^16, batch 256
     hipStream_t st = (hipStream_t)stream;
     int s = precision == GPUFFT_PREC_F64 ? make_tables<double>(pl, st) : make_tables<float>(pl, st);
     if (s) {
@@ -245,6 +263,21 @@ int gpufft_polymul_plan_create(gpufft_polymul_plan* out, int ring_power, int pre
             release(pl);
             return fail_hip(e, "gpufft_polymul_plan_create: hipMalloc scratch");
         }
+    }
+    if (precision == GPUFFT_PREC_F64 && pl->SB > 0) {
+        hipError_t e = hipMalloc((void**)&pl->arrive, sizeof(int) * (size_t)max_batch);
+        if (e != hipSuccess) {
+            release(pl);
+            return fail_hip(e, "gpufft_polymul_plan_create: hipMalloc counters");
+        }
+        hipLaunchKernelGGL(zero_ints, dim3((unsigned)((max_batch + 255) / 256)), dim3(256), 0, st,
+                           pl->arrive, max_batch);
+    }
+    // tables and counters are ready before the plan is used on any stream
+    hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        release(pl);
+        return fail_hip(e, "gpufft_polymul_plan_create: table generation");
     }
     *out = pl;
     return ok();
@@ -275,9 +308,12 @@ int gpufft_polymul_f64_phases(gpufft_polymul_plan plan, const double* a, const d
         const double* pa = a + b0 * N;
         const double* pb = b + b0 * N;
         double* po = out + b0 * N;
+        // in-kernel combine: one launch does both phases (PHASE_TRANSFORM
+        // runs it, PHASE_COMBINE alone does nothing)
+        const int cmb = plan->cmb && plan->SB > 0;
         if (phases & GPUFFT_PHASE_TRANSFORM)
-            if (int s = launch_products<double, PL_FOLD>(plan, pa, pb, po, nb, st, who)) return s;
-        if (plan->SB > 0 && (phases & GPUFFT_PHASE_COMBINE)) {
+            if (int s = launch_products<double, PL_FOLD>(plan, pa, pb, po, nb, cmb, st, who)) return s;
+        if (plan->SB > 0 && !cmb && (phases & GPUFFT_PHASE_COMBINE)) {
             // top SB inverse stages + scale + untwist + unfold
             PassArgs<double> pa2{};
             pa2.n = plan->n;
@@ -328,7 +364,7 @@ int gpufft_polymul_i32_phases(gpufft_polymul_plan plan, const int32_t* a, const 
         const int nb = std::min(plan->max_batch, batch - b0);
         if (phases & GPUFFT_PHASE_TRANSFORM)
             if (int s = launch_products<float, PL_FOLD_LIMB>(plan, a + b0 * N, b + b0 * N, nullptr,
-                                                             nb, st, who))
+                                                             nb, 0, st, who))
                 return s;
         if (!(phases & GPUFFT_PHASE_COMBINE)) continue;
         const int nthreads_total = nb << plan->S;
@@ -385,6 +421,22 @@ int gpufft_polymul_plan_set_radix_bits(gpufft_polymul_plan plan, int radix_bits)
 int gpufft_polymul_plan_get_radix_bits(gpufft_polymul_plan plan)
 {
     return plan ? plan->rb : -1;
+}
+
+int gpufft_polymul_plan_set_combine(gpufft_polymul_plan plan, int in_kernel)
+{
+    const char* who = "gpufft_polymul_plan_set_combine";
+    if (!plan) return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": plan is NULL");
+    if (in_kernel != 0 && in_kernel != 1)
+        return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": in_kernel must be 0 or 1");
+    // only FP64 plans with more than one workgroup per product have the choice
+    plan->cmb = plan->precision == GPUFFT_PREC_F64 && plan->SB > 0 ? in_kernel : 0;
+    return ok();
+}
+
+int gpufft_polymul_plan_get_combine(gpufft_polymul_plan plan)
+{
+    return plan ? plan->cmb : -1;
 }
 
 }  // extern "C"

@@ -88,6 +88,8 @@ _SIGS = {
     "gpufft_batched_execute": [_P, _P, _P, _P, _P, _P],
     "gpufft_batched_graph_launch": [_P, _P, _P, _P, _P, _P],
     "gpufft_polymul_plan_get_radix_bits": [_P],
+    "gpufft_polymul_plan_set_combine": [_P, _I],
+    "gpufft_polymul_plan_get_combine": [_P],
     "gpufft_polymul_i32_phases": [_P, _P, _P, _P, _I, _P, _I, _P],
 }
 
@@ -239,6 +241,16 @@ class PolymulPlan:
     @radix_bits.setter
     def radix_bits(self, rb):
         check(_lib.gpufft_polymul_plan_set_radix_bits(self.handle, rb))
+
+    @property
+    def combine_in_kernel(self):
+        """FP64 plans of ring_power >= 15: last inverse stages in the product
+        kernel (True) or in a second launch (False, default)."""
+        return bool(_lib.gpufft_polymul_plan_get_combine(self.handle))
+
+    @combine_in_kernel.setter
+    def combine_in_kernel(self, v):
+        check(_lib.gpufft_polymul_plan_set_combine(self.handle, 1 if v else 0))
 
     @property
     def scratch_bytes(self):

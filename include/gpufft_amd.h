@@ -174,6 +174,15 @@ size_t gpufft_polymul_plan_scratch_bytes(gpufft_polymul_plan plan);
  * sets it per ring size and device. */
 int gpufft_polymul_plan_set_radix_bits(gpufft_polymul_plan plan, int radix_bits);
 int gpufft_polymul_plan_get_radix_bits(gpufft_polymul_plan plan);
+/* GPUFFT_PREC_F64 plans whose product spans several workgroups (ring_power
+ * >= 15): in_kernel = 1 runs the last inverse stages inside the product
+ * kernel, in the workgroup that finishes a product last (one launch); 0
+ * (default) runs them as a second, streaming launch.  On MI355X the second
+ * launch is faster: the in-kernel form's device-scope release/acquire write
+ * back and invalidate L2, which the product kernel's input reads rely on.
+ * Other plans ignore the setting (get returns 0).  Same results either way. */
+int gpufft_polymul_plan_set_combine(gpufft_polymul_plan plan, int in_kernel);
+int gpufft_polymul_plan_get_combine(gpufft_polymul_plan plan);
 int gpufft_polymul_f64(gpufft_polymul_plan plan, const double* a, const double* b, double* out,
                        int batch, void* stream);
 int gpufft_polymul_i32(gpufft_polymul_plan plan, const int32_t* a, const int32_t* b, int64_t* out,
@@ -182,9 +191,10 @@ int gpufft_polymul_i32(gpufft_polymul_plan plan, const int32_t* a, const int32_t
  * schedule or time them apart: GPUFFT_PHASE_TRANSFORM runs the forward
  * transforms, products and inverse sub-transforms into the plan's scratch,
  * GPUFFT_PHASE_COMBINE the last inverse stages and the output (for plans
- * whose whole product fits one workgroup the first phase writes the output
- * and the second does nothing).  A combine must follow the transform of the
- * same inputs on the same plan, with batch <= max_batch. */
+ * whose whole product fits one workgroup, and for FP64 plans with the
+ * in-kernel combine, the first phase writes the output and the second does
+ * nothing).  A combine must follow the transform of the same inputs on the
+ * same plan, with batch <= max_batch. */
 #define GPUFFT_PHASE_TRANSFORM 1
 #define GPUFFT_PHASE_COMBINE 2
 #define GPUFFT_PHASE_ALL 3

@@ -48,16 +48,21 @@ def test_gpu_tables_match_reference_layout(n_power):
 
 
 # ---- FP64 fused product ----
+@pytest.mark.parametrize("combine", ["in_kernel", "pass"])
 @pytest.mark.parametrize("rb", [3, 4])
 @pytest.mark.parametrize("ring_power", [7, 8, 12, 14, 15, 16, 17])
-def test_polymul_f64_exact(ring_power, rb):
+def test_polymul_f64_exact(ring_power, rb, combine):
     G = _g()
     import torch
+    if combine == "pass" and ring_power < 15:
+        pytest.skip("one workgroup per product: no combine step")
     N = 1 << ring_power
     batch = 11  # not a multiple of the 8-XCD block grouping
     a, b = _operands(N, batch, SEED_BASE + 3 + ring_power)
     plan = G.PolymulPlan(ring_power, G.PREC_F64, max_batch=batch)
     plan.radix_bits = rb
+    plan.combine_in_kernel = combine == "in_kernel"
+    assert plan.combine_in_kernel == (combine == "in_kernel" and ring_power >= 15)
     out = torch.empty(batch * N, dtype=torch.float64, device="cuda")
     plan.multiply_f64(to_dev(a.astype(np.float64).reshape(-1)), to_dev(b.astype(np.float64).reshape(-1)),
                       out, batch)
@@ -74,14 +79,17 @@ def test_polymul_f64_exact(ring_power, rb):
     assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
 
 
-def test_polymul_f64_chunks_and_reuse():
+@pytest.mark.parametrize("in_kernel", [True, False])
+def test_polymul_f64_chunks_and_reuse(in_kernel):
     """batch larger than max_batch runs in chunks; a second call on the same
-    plan with a smaller batch gives the same products."""
+    plan with a smaller batch gives the same products (the in-kernel combine's
+    arrival counters are back at zero after every launch)."""
     G = _g()
     import torch
     rp, N = 15, 1 << 15
     a, b = _operands(N, 20, SEED_BASE + 77)
     plan = G.PolymulPlan(rp, G.PREC_F64, max_batch=8)
+    plan.combine_in_kernel = in_kernel
     da, db = to_dev(a.astype(np.float64).reshape(-1)), to_dev(b.astype(np.float64).reshape(-1))
     out = torch.empty(20 * N, dtype=torch.float64, device="cuda")
     plan.multiply_f64(da, db, out, 20)
@@ -90,6 +98,26 @@ def test_polymul_f64_chunks_and_reuse():
     out2 = torch.empty(3 * N, dtype=torch.float64, device="cuda")
     plan.multiply_f64(da, db, out2, 3)
     assert np.array_equal(np.round(to_host(out2)).astype(np.int64).reshape(3, N), exact[:3])
+
+
+def test_polymul_f64_combine_forms_agree():
+    """in-kernel combine and the separate combine pass give the same unrounded
+    values to FP64 rounding, call after call."""
+    G = _g()
+    import torch
+    rp, N, batch = 16, 1 << 16, 9
+    a, b = _operands(N, batch, SEED_BASE + 91)
+    da, db = to_dev(a.astype(np.float64).reshape(-1)), to_dev(b.astype(np.float64).reshape(-1))
+    plan = G.PolymulPlan(rp, G.PREC_F64, max_batch=batch)
+    outs = []
+    for in_kernel in (True, False, True):
+        plan.combine_in_kernel = in_kernel
+        o = torch.empty(batch * N, dtype=torch.float64, device="cuda")
+        plan.multiply_f64(da, db, o, batch)
+        outs.append(to_host(o))
+    scale = np.max(np.abs(outs[1]))
+    assert np.max(np.abs(outs[0] - outs[1])) <= 1e-13 * scale
+    assert np.array_equal(outs[0], outs[2])
 
 
 # ---- FP32-split fused product ----
