@@ -41,7 +41,24 @@ def parse():
     p.add_argument("--rb-f64", type=int, default=0, help="radix bits of the FP64 kernel (0: tune)")
     p.add_argument("--rb-f32", type=int, default=0, help="radix bits of the FP32 kernel (0: tune)")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="only run 3 device copies of 512 MiB (reference kernel for HBM counters)")
     return p.parse_args()
+
+
+def calibrate_copy():
+    """Three plain device-to-device copies of 512 MiB: 536870912 bytes read and
+    536870912 bytes written per copy, the known amounts against which the HBM
+    traffic counters (FETCH_SIZE, WRITE_SIZE) are scaled."""
+    import torch
+    n = 64 << 20
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "copy", "bytes_read_per_copy": 8 * n,
+                      "bytes_written_per_copy": 8 * n, "copies": 3}))
 
 
 def timed(fn, stream, steps, warmup):
@@ -62,6 +79,9 @@ def timed(fn, stream, steps, warmup):
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; run with --gpus 1",
                           "n_gpus": args.gpus}))
