@@ -29,7 +29,7 @@
 namespace gfa {
 
 #ifndef GFA_LOAD_UNROLL
-#define GFA_LOAD_UNROLL 2
+#define GFA_LOAD_UNROLL 1
 #endif
 
 enum ProdLoad {
@@ -72,25 +72,48 @@ __device__ __forceinline__ int limb_digit(int v, int w, int i, int k, long long 
     return (int)((u >> (w * i)) & ((1LL << w) - 1)) - (1 << (w - 1));
 }
 
-// Folded input (x_P, x_{P+L}) of this product's operand, as T.
+// Positions per load of an operand: 16-byte loads of 2 doubles (FP64 form),
+// 8-byte loads of 2 int32 (limb form; the limb kernels have no registers
+// to spare for wider groups).
+template <typename T, int PL> constexpr int in_vec() { return 2; }
+
+// Folded inputs of V = in_vec consecutive positions P .. P+V-1 of this
+// product's operand, as T: re[v] = x_{P+v}, im[v] = x_{P+v+L}; two vector
+// buffer loads (P and L are multiples of V).  Wide lanes cut the address and
+// tag work per byte, which bounds the gather of the top network (each input
+// is read by every workgroup of its product).
 template <typename T, int PL>
-__device__ __forceinline__ void prod_input(rsrc_t x, int L, int w, int limb, int k, long long o,
-                                           int P, T& re, T& im)
+__device__ __forceinline__ void prod_input_vec(rsrc_t x, int L, int w, int limb, int k, long long o,
+                                               int P, T* re, T* im)
 {
     if constexpr (PL == PL_FOLD) {
-        const unsigned o0 = (unsigned)P * sizeof(T), o1 = (unsigned)(P + L) * sizeof(T);
-        if constexpr (sizeof(T) == 8) {
-            re = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(x, o0, 0, 0));
-            im = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(x, o1, 0, 0));
-        } else {
-            re = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(x, o0, 0, 0));
-            im = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(x, o1, 0, 0));
-        }
+        static_assert(sizeof(T) == 8, "the FP64 form reads doubles");
+        const u32x4 r = __builtin_amdgcn_raw_buffer_load_b128(x, (unsigned)P * 8u, 0, 0);
+        const u32x4 i = __builtin_amdgcn_raw_buffer_load_b128(x, (unsigned)(P + L) * 8u, 0, 0);
+        __builtin_memcpy(re, &r, 16);
+        __builtin_memcpy(im, &i, 16);
     } else {
-        const int v0 = (int)__builtin_amdgcn_raw_buffer_load_b32(x, (unsigned)P * 4u, 0, 0);
-        const int v1 = (int)__builtin_amdgcn_raw_buffer_load_b32(x, (unsigned)(P + L) * 4u, 0, 0);
-        re = (T)limb_digit(v0, w, limb, k, o);
-        im = (T)limb_digit(v1, w, limb, k, o);
+        const u32x2 r = __builtin_amdgcn_raw_buffer_load_b64(x, (unsigned)P * 4u, 0, 0);
+        const u32x2 i = __builtin_amdgcn_raw_buffer_load_b64(x, (unsigned)(P + L) * 4u, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 2; v++) {
+            re[v] = (T)limb_digit((int)r[v], w, limb, k, o);
+            im[v] = (T)limb_digit((int)i[v], w, limb, k, o);
+        }
+    }
+}
+
+// V consecutive twist factors tw[c0 .. c0+V-1] (c0 a multiple of V), in
+// 16-byte loads (one complex double, or two complex floats, per load).
+template <typename T, int V>
+__device__ __forceinline__ void twist_vec(rsrc_t tw, int c0, cplx<T>* t)
+{
+    constexpr int PER = 16 / (int)sizeof(cplx<T>);
+    static_assert(V % PER == 0, "whole 16-byte twist loads");
+#pragma unroll
+    for (int v = 0; v < V; v += PER) {
+        const u32x4 r = __builtin_amdgcn_raw_buffer_load_b128(tw, (unsigned)(c0 + v) * (unsigned)sizeof(cplx<T>), 0, 0);
+        __builtin_memcpy(t + v, &r, 16);
     }
 }
 
@@ -150,30 +173,36 @@ __device__ __forceinline__ void load_sub(cplx<T>* lds, const ProdArgs<T>& a, rsr
 {
     constexpr int NT = (1 << S) >> E;
     constexpr int Q = 1 << SB;
-    constexpr int U = GFA_LOAD_UNROLL;  // elements in flight per thread
+    constexpr int V = in_vec<T, PL>();
+    constexpr int NG = (1 << E) / V;  // groups of V consecutive positions per thread
+    constexpr int U = GFA_LOAD_UNROLL;  // groups in flight per thread
+    static_assert(NG % U == 0, "load unroll must divide the groups per thread");
     const int L = 1 << a.n;
 #pragma unroll 1
-    for (int i0 = 0; i0 < (1 << E); i0 += U) {
-        T re[U][Q], im[U][Q];
-        cplx<T> tlo[U];
+    for (int g0 = 0; g0 < NG; g0 += U) {
+        T re[U][Q][V], im[U][Q][V];
+        cplx<T> tl[U][V];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const int c = tid + (i0 + u) * NT;
-            tlo[u] = bldc<T>(tw, c);
+            const int c0 = (tid + (g0 + u) * NT) * V;
+            twist_vec<T, V>(tw, c0, tl[u]);
 #pragma unroll
             for (int j = 0; j < Q; j++)
-                prod_input<T, PL>(x, L, w, limb, k, o, (j << S) | c, re[u][j], im[u][j]);
+                prod_input_vec<T, PL>(x, L, w, limb, k, o, (j << S) | c0, re[u][j], im[u][j]);
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const int c = tid + (i0 + u) * NT;
-            T ax = T(0), ay = T(0);
+            const int c0 = (tid + (g0 + u) * NT) * V;
 #pragma unroll
-            for (int j = 0; j < Q; j++) {
-                ax = ax + C[j].x * re[u][j] - C[j].y * im[u][j];
-                ay = ay + C[j].x * im[u][j] + C[j].y * re[u][j];
+            for (int v = 0; v < V; v++) {
+                T ax = T(0), ay = T(0);
+#pragma unroll
+                for (int j = 0; j < Q; j++) {
+                    ax = ax + C[j].x * re[u][j][v] - C[j].y * im[u][j][v];
+                    ay = ay + C[j].x * im[u][j][v] + C[j].y * re[u][j][v];
+                }
+                lds[swz(c0 + v)] = tl[u][v] * mk<T>(ax, ay);
             }
-            lds[swz(c)] = tlo[u] * mk<T>(ax, ay);
         }
     }
 }
