@@ -41,6 +41,11 @@ def parse():
     p.add_argument("--rb-f64", type=int, default=0, help="radix bits of the FP64 kernel (0: tune)")
     p.add_argument("--rb-f32", type=int, default=0, help="radix bits of the FP32 kernel (0: tune)")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--surface", action="store_true",
+                   help="time the drop-in entry points on the reference's benchmark axes "
+                        "(one JSON line per point) instead of the headline workload")
+    p.add_argument("--surface-logn", default="12-23", help="ring size logN range of --surface")
+    p.add_argument("--surface-dtype", default="f64", choices=["f64", "f32"])
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only run 3 device copies of 512 MiB (reference kernel for HBM counters)")
     return p.parse_args()
@@ -77,10 +82,133 @@ def timed(fn, stream, steps, warmup):
     return e0.elapsed_time(e1) / steps
 
 
+SURFACE = ("fft_forward", "fft_inverse", "fft_c2c_mult", "fft_r2r_mult", "ffnt_r2r_mult")
+
+
+def surface(args):
+    """The reference's nvbench programs (benchmark/bench_gpu_fft.cu,
+    bench_gpu_fft_C_C_mult.cu, bench_gpu_fft_R_R_mult.cu,
+    bench_gpu_ffnt_R_R_mult.cu): ring size logN 12..23, batch 1,
+    X^N-1 transforms of length 2^(logN+1) (FFNT: n_power = logN), the
+    product benches timing a FORWARD call on 2 batches then an INVERSE call
+    with multiplication.  Same calls through the drop-in entry points; one
+    JSON line per (bench, logN): `ms` is GPU time per call (median over
+    --steps replays of a HIP graph of 20 captured calls, no host launch
+    cost, as nvbench's GPU time), `eager_ms` the same call launched from
+    Python with events around it."""
+    import numpy as np
+    import torch
+
+    import gpufft_amd as G
+    lo, hi = (int(x) for x in args.surface_logn.split("-"))
+    f32 = args.surface_dtype == "f32"
+    rdt = torch.float32 if f32 else torch.float64
+    cdt = torch.complex64 if f32 else torch.complex128
+    esz = 4 if f32 else 8
+    prec = "f32" if f32 else "f64"
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(device=dev)
+    sid = stream.cuda_stream
+    batch = 1
+    gen = torch.Generator(device="cpu").manual_seed(1234)
+
+    def rnd_c(n):
+        return torch.complex(torch.rand(n, generator=gen, dtype=rdt) - 0.5,
+                             torch.rand(n, generator=gen, dtype=rdt) - 0.5).to(dev)
+
+    reps = 20
+
+    def median_ms(fn):
+        """(GPU ms per call from replays of a graph of `reps` captured calls,
+        eager ms per call with events around each call)."""
+        for _ in range(max(1, args.warmup)):
+            fn()
+        torch.cuda.synchronize()
+        eager = []
+        for _ in range(args.steps):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            e1.synchronize()
+            eager.append(e0.elapsed_time(e1))
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=stream):
+            for _ in range(reps):
+                fn()
+        g.replay()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(args.steps):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            g.replay()
+            e1.record(stream)
+            e1.synchronize()
+            ts.append(e0.elapsed_time(e1) / reps)
+        del g
+        return float(np.median(ts)), float(np.median(eager))
+
+    # a graph replays the captured calls on their capture stream
+    with torch.cuda.stream(stream):
+        for logn in range(lo, hi + 1):
+            rs = 1 << logn
+            n_power = logn + 1
+            tab = G.generate_table(G.TABLE_FFT, n_power - 1, False, prec=prec, stream=sid)
+            itab = G.generate_table(G.TABLE_FFT, n_power - 1, True, prec=prec, stream=sid)
+            cf = G.config(n_power, G.FORWARD, stream=sid)
+            ci = G.config(n_power, G.INVERSE, mod_inverse=complex(1.0, 1.0), stream=sid)
+            for name in SURFACE:
+                if name in ("fft_forward", "fft_inverse"):
+                    x = rnd_c(2 * rs * batch)
+                    cfg = cf if name == "fft_forward" else ci
+                    t = tab if name == "fft_forward" else itab
+                    ms = median_ms(lambda: G.GPU_FFT(x, t, cfg, batch))
+                    alg = 2 * (2 * rs) * 2 * esz * batch      # complex in + out, length 2^(logN+1)
+                    del x
+                elif name == "fft_c2c_mult":
+                    x = rnd_c(2 * rs * 2 * batch)
+                    ms = median_ms(lambda: (G.GPU_FFT(x, tab, cf, 2 * batch),
+                                            G.GPU_FFT(x, itab, ci, batch, True)))
+                    alg = 3 * (2 * rs) * 2 * esz * batch     # two operands in, product out
+                    del x
+                elif name == "fft_r2r_mult":
+                    r = (torch.rand(2 * rs * 2 * batch, generator=gen, dtype=rdt) - 0.5).to(dev)
+                    tmp = torch.empty(2 * rs * 2 * batch, dtype=cdt, device=dev)
+                    ms = median_ms(lambda: (G.GPU_FFT_R(r, tmp, tab, cf, 2 * batch),
+                                            G.GPU_FFT_R(r, tmp, itab, ci, batch, True)))
+                    alg = 3 * (2 * rs) * esz * batch
+                    del r, tmp
+                else:  # ffnt_r2r_mult: ring size 2^logN, FFNT n_power = logN
+                    r = (torch.rand(rs * 2 * batch, generator=gen, dtype=rdt) - 0.5).to(dev)
+                    tmp = torch.empty(rs // 2 * 2 * batch, dtype=cdt, device=dev)
+                    ftw = G.generate_table(G.TABLE_FFNT_TWIST, logn - 1, False, prec=prec, stream=sid)
+                    fut = G.generate_table(G.TABLE_FFNT_TWIST, logn - 1, True, prec=prec, stream=sid)
+                    frt = G.generate_table(G.TABLE_FFNT_ROOT, logn - 1, False, prec=prec, stream=sid)
+                    fir = G.generate_table(G.TABLE_FFNT_ROOT, logn - 1, True, prec=prec, stream=sid)
+                    cff = G.config(logn, G.FORWARD, stream=sid)
+                    cfi = G.config(logn, G.INVERSE, mod_inverse=complex(1.0, 1.0), stream=sid)
+                    ms = median_ms(lambda: (G.GPU_FFNT(r, tmp, ftw, frt, cff, 2 * batch),
+                                            G.GPU_FFNT(r, tmp, fut, fir, cfi, batch, True)))
+                    alg = 3 * rs * esz * batch
+                    del r, tmp
+                ms, eager_ms = ms
+                print(json.dumps({"bench": name, "ring_size_logn": logn, "batch": batch,
+                                  "dtype": args.surface_dtype, "ms": ms, "eager_ms": eager_ms,
+                                  "alg_bytes": alg, "alg_GBps": alg / (ms * 1e-3) / 1e9}),
+                      flush=True)
+            torch.cuda.synchronize()
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.surface:
+        surface(args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; run with --gpus 1",
