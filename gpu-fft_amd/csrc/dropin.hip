@@ -4,6 +4,7 @@
 // cfg->stream (one launch in total up to 2^12 double / 2^13 float points).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <string>
 
 #include "status.hpp"
@@ -197,9 +198,115 @@ int special_split(void* inout, const void* table, const gpufft_config* cfg, int 
     return run_transform<float>(mode, n, batch, 0, (const cplx<float>*)table, io, st, who);
 }
 
+// CKKS slot <-> coefficient steps around the special FFT.  l = 2^n slots,
+// ring N = 2l: coefficient i pairs with slot-transform entry bitrev(i) (the
+// special inverse leaves its output bit-reversed, the special forward wants
+// its input bit-reversed; fft.cu:727-926), real parts give m[0..l), imaginary
+// parts m[l..2l).
+__global__ void ckks_round_kernel(const cplx<double>* u, long long* m, int n, long long total,
+                                  double delta)
+{
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= total) return;
+    const long long l = 1LL << n;
+    const long long b = t >> n;
+    const int i = (int)(t & (l - 1));
+    const int br = (int)(__brev((unsigned)i) >> (32 - n));
+    const cplx<double> z = u[b * l + br];
+    m[2 * b * l + i] = llround(z.x * delta);
+    m[2 * b * l + i + l] = llround(z.y * delta);
+}
+
+__global__ void ckks_unround_kernel(const long long* m, cplx<double>* v, int n, long long total,
+                                    double delta)
+{
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= total) return;
+    const long long l = 1LL << n;
+    const long long b = t >> n;
+    const int i = (int)(t & (l - 1));
+    const int br = (int)(__brev((unsigned)i) >> (32 - n));
+    v[b * l + br] = mk<double>((double)m[2 * b * l + i] / delta, (double)m[2 * b * l + i + l] / delta);
+}
+
+int check_ckks(int n_power, double delta, int batch, const char* who)
+{
+    if (n_power < GPUFFT_MIN_N_POWER || n_power > GPUFFT_MAX_N_POWER)
+        return fail(GPUFFT_ERR_INVALID_ARGUMENT, "Invalid n_power range!");
+    if (!(delta > 0.0) || !std::isfinite(delta))
+        return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": scale must be finite and > 0");
+    return check_batch(batch, who);
+}
+
+int ckks_encode(const void* slots, int64_t* coeffs, const void* inv_table, int n_power,
+                double delta, int batch, void* work, void* stream, const char* who)
+{
+    if (int s = check_ckks(n_power, delta, batch, who)) return s;
+    if (!slots || !coeffs || !inv_table || !work)
+        return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": NULL buffer");
+    const long long L = 1LL << n_power;
+    hipStream_t st = (hipStream_t)stream;
+    TransformIO<double> io{};
+    io.lk = LD_CPLX;
+    io.src = slots;
+    io.src_stride = L;
+    io.sk = ST_CPLX;
+    io.dst = work;
+    io.dst_stride = L;
+    io.work = (cplx<double>*)work;
+    io.do_scale = true;
+    io.scale = mk<double>(1.0 / (double)L, 0.0);
+    if (int s = run_transform<double>(MODE_SPE_INV, n_power, batch, 0, (const cplx<double>*)inv_table,
+                                      io, st, who))
+        return s;
+    const long long total = (long long)batch << n_power;
+    hipLaunchKernelGGL(ckks_round_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
+                       (const cplx<double>*)work, (long long*)coeffs, n_power, total, delta);
+    GFA_CHECK_LAUNCH(who);
+    return ok();
+}
+
+int ckks_decode(const int64_t* coeffs, void* slots, const void* table, int n_power, double delta,
+                int batch, void* stream, const char* who)
+{
+    if (int s = check_ckks(n_power, delta, batch, who)) return s;
+    if (!slots || !coeffs || !table)
+        return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": NULL buffer");
+    const long long L = 1LL << n_power;
+    hipStream_t st = (hipStream_t)stream;
+    const long long total = (long long)batch << n_power;
+    hipLaunchKernelGGL(ckks_unround_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
+                       (const long long*)coeffs, (cplx<double>*)slots, n_power, total, delta);
+    GFA_CHECK_LAUNCH(who);
+    cplx<double>* x = (cplx<double>*)slots;
+    TransformIO<double> io{};
+    io.lk = LD_CPLX;
+    io.src = x;
+    io.src_stride = L;
+    io.sk = ST_CPLX;
+    io.dst = x;
+    io.dst_stride = L;
+    io.work = x;
+    return run_transform<double>(MODE_SPE_FWD, n_power, batch, 0, (const cplx<double>*)table, io, st,
+                                 who);
+}
+
 }  // namespace
 
 extern "C" {
+
+int gpufft_ckks_encode_f64(const void* slots, int64_t* coeffs, const void* inv_table, int n_power,
+                           double scale, int batch, void* work, void* stream)
+{
+    return ckks_encode(slots, coeffs, inv_table, n_power, scale, batch, work, stream,
+                       "gpufft_ckks_encode_f64");
+}
+
+int gpufft_ckks_decode_f64(const int64_t* coeffs, void* slots, const void* table, int n_power,
+                           double scale, int batch, void* stream)
+{
+    return ckks_decode(coeffs, slots, table, n_power, scale, batch, stream, "gpufft_ckks_decode_f64");
+}
 
 int gpufft_special_fft_split_f32(void* inout, const void* root_table, const gpufft_config* cfg,
                                  int batch, void* work)

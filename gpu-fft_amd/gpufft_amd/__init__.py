@@ -91,6 +91,8 @@ _SIGS = {
     "gpufft_polymul_plan_set_combine": [_P, _I],
     "gpufft_polymul_plan_get_combine": [_P],
     "gpufft_polymul_i32_phases": [_P, _P, _P, _P, _I, _P, _I, _P],
+    "gpufft_ckks_encode_f64": [_P, _P, _P, _I, ctypes.c_double, _I, _P, _P],
+    "gpufft_ckks_decode_f64": [_P, _P, _P, _I, ctypes.c_double, _I, _P],
 }
 
 
@@ -204,6 +206,20 @@ def GPU_Special_FFT(inout, root_table, cfg, batch_size):
 
 # ---- GPU table generation (reference layouts) ----
 TABLE_FFT, TABLE_FFNT_ROOT, TABLE_FFNT_TWIST, TABLE_SPECIAL = 0, 1, 2, 3
+
+
+def ckks_encode(slots, coeffs, inv_table, n_power, scale, batch, work, stream=None):
+    """CKKS encode: complex slots [batch][2^n_power] -> int64 coefficients
+    [batch][2^(n_power+1)] (special inverse FFT, bit reversal, x scale, round)."""
+    check(_lib.gpufft_ckks_encode_f64(_ptr(slots), _ptr(coeffs), _ptr(inv_table), n_power,
+                                      float(scale), batch, _ptr(work), _stream(stream)))
+
+
+def ckks_decode(coeffs, slots, table, n_power, scale, batch, stream=None):
+    """CKKS decode: int64 coefficients -> complex slots (/ scale, bit reversal,
+    special forward FFT)."""
+    check(_lib.gpufft_ckks_decode_f64(_ptr(coeffs), _ptr(slots), _ptr(table), n_power,
+                                      float(scale), batch, _stream(stream)))
 
 
 def generate_table(kind, n_power, inverse=False, prec="f64", stream=None):

@@ -204,6 +204,26 @@ int gpufft_polymul_i32_phases(gpufft_polymul_plan plan, const int32_t* a, const 
                               int64_t* out, int batch, double* err_stats, int phases,
                               void* stream);
 
+/* ---- CKKS encode / decode around the special FFT (no counterpart in the
+ * reference, which documents these caller steps around GPU_Special_FFT in
+ * src/include/gpufft/fft.cuh:134-180 without implementing them) ----
+ * l = 2^n_power slots (n_power 1..24), ring N = 2l, scale = Delta > 0.
+ *   encode: slots (complex double [batch][l], read only) -> coeffs (int64
+ *           [batch][2l]): u = special inverse FFT of the slots with
+ *           inv_table (GPUFFT_TABLE_SPECIAL, inverse) and mod_inverse 1/l;
+ *           coeffs[i] = llround(Re u[bitrev(i)] * scale),
+ *           coeffs[i + l] = llround(Im u[bitrev(i)] * scale).
+ *           work: complex double [batch][l].  |u * scale| must stay below
+ *           2^63 (not checked).
+ *   decode: coeffs -> slots: slots[bitrev(i)] = (coeffs[i] / scale,
+ *           coeffs[i + l] / scale), then the special forward FFT in place
+ *           with table (GPUFFT_TABLE_SPECIAL, forward), so that
+ *           slots[j] = m(zeta_2N^(5^j mod 2N)) / scale. */
+int gpufft_ckks_encode_f64(const void* slots, int64_t* coeffs, const void* inv_table, int n_power,
+                           double scale, int batch, void* work, void* stream);
+int gpufft_ckks_decode_f64(const int64_t* coeffs, void* slots, const void* table, int n_power,
+                           double scale, int batch, void* stream);
+
 /* ---- batched products of uneven ring sizes, one persistent launch ----
  * (no counterpart in the reference, which needs one GPU_FFT/GPU_FFNT call
  * pair per ring size).  Each product of the list is

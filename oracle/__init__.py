@@ -63,6 +63,8 @@ def _declare(L):
         getattr(L, "orc_special_inverse_" + s).argtypes = [_P, _I, _I, _P, T, T]
     for name in ("orc_schoolbook_linear", "orc_schoolbook_negacyclic", "orc_schoolbook_cyclic"):
         getattr(L, name).argtypes = [_I, _P, _P, _P]
+    L.orc_ckks_encode_f64.argtypes = [_P, _I, _I, _P, ctypes.c_double, _P]
+    L.orc_ckks_decode_f64.argtypes = [_P, _I, _I, _P, ctypes.c_double, _P]
     L.orc_ntt_negacyclic.argtypes = [_I, _I, _P, _P, _P]
     L.orc_ntt_cyclic.argtypes = [_I, _I, _P, _P, _P]
     L.orc_mix64.argtypes = [ctypes.c_uint64]
@@ -199,6 +201,26 @@ def special_forward(data, n_power, batch, table, prec="f64"):
     getattr(lib(), "orc_special_forward_" + prec)(_ptr(data), n_power, batch,
                                                   _ptr(np.ascontiguousarray(table, _cdt(prec))))
     return data
+
+
+def ckks_encode(slots, n_power, batch, inv_table, delta):
+    """int64 [batch][2l] coefficients of the slots (oracle.h: CKKS encode)."""
+    z = np.ascontiguousarray(slots, np.complex128)
+    out = np.empty(batch * (2 << n_power), np.int64)
+    lib().orc_ckks_encode_f64(_ptr(z), n_power, batch,
+                              _ptr(np.ascontiguousarray(inv_table, np.complex128)),
+                              ctypes.c_double(delta), _ptr(out))
+    return out
+
+
+def ckks_decode(coeffs, n_power, batch, table, delta):
+    """complex [batch][l] slots of int64 coefficients (oracle.h: CKKS decode)."""
+    m = np.ascontiguousarray(coeffs, np.int64)
+    out = np.empty(batch << n_power, np.complex128)
+    lib().orc_ckks_decode_f64(_ptr(m), n_power, batch,
+                              _ptr(np.ascontiguousarray(table, np.complex128)),
+                              ctypes.c_double(delta), _ptr(out))
+    return out
 
 
 def special_inverse(data, n_power, batch, table, scale, prec="f64"):

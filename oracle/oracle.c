@@ -507,6 +507,37 @@ void orc_cpu_ffnt_mult_batch_f64(int N, int count, const int64_t *a, const int64
 DEF_GPU_SEM(double, f64)
 DEF_GPU_SEM(float, f32)
 
+/* CKKS encode / decode: the steps a caller of GPU_Special_FFT performs around
+ * it (fft.cuh:134-180); see oracle.h. */
+void orc_ckks_encode_f64(const double *slots, int n_power, int batch, const double *inv_table,
+                         double delta, int64_t *coeffs)
+{
+    size_t l = (size_t)1 << n_power;
+    double *u = (double *)malloc(sizeof(double) * 2 * l * (size_t)batch);
+    memcpy(u, slots, sizeof(double) * 2 * l * (size_t)batch);
+    orc_special_inverse_f64(u, n_power, batch, inv_table, 1.0 / (double)l, 0.0);
+    for (int b = 0; b < batch; b++)
+        for (size_t i = 0; i < l; i++) {
+            const double *z = u + 2 * ((size_t)b * l + (size_t)orc_bitreverse((int)i, n_power));
+            coeffs[(size_t)b * 2 * l + i] = llround(z[0] * delta);
+            coeffs[(size_t)b * 2 * l + i + l] = llround(z[1] * delta);
+        }
+    free(u);
+}
+
+void orc_ckks_decode_f64(const int64_t *coeffs, int n_power, int batch, const double *table,
+                         double delta, double *slots)
+{
+    size_t l = (size_t)1 << n_power;
+    for (int b = 0; b < batch; b++)
+        for (size_t i = 0; i < l; i++) {
+            double *v = slots + 2 * ((size_t)b * l + (size_t)orc_bitreverse((int)i, n_power));
+            v[0] = (double)coeffs[(size_t)b * 2 * l + i] / delta;
+            v[1] = (double)coeffs[(size_t)b * 2 * l + i + l] / delta;
+        }
+    orc_special_forward_f64(slots, n_power, batch, table);
+}
+
 /* ------------------------------------------------------------------------ */
 /* exact integer products                                                   */
 /* ------------------------------------------------------------------------ */

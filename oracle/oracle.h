@@ -82,6 +82,19 @@ ORC_DECL(double, f64)
 ORC_DECL(float, f32)
 #undef ORC_DECL
 
+/* ---- CKKS encode / decode around the special FFT ----
+ * The reference documents these caller steps (src/include/gpufft/fft.cuh:
+ * 134-180, the HEAAN-style fftSpecialInv / fftSpecial pair) without code.
+ * l = 2^n_power slots, ring N = 2l, scale delta:
+ *   encode: u = special_inverse(z) (bit-reversed order, scaled 1/l);
+ *           m[i] = llround(Re u[bitrev(i)] * delta), m[i + l] = llround(Im ... * delta)
+ *   decode: v[bitrev(i)] = (m[i] / delta, m[i + l] / delta); z = special_forward(v)
+ * so decode(m)_j = m(zeta_{2N}^(5^j mod 2N)) / delta. */
+void orc_ckks_encode_f64(const double *slots, int n_power, int batch, const double *inv_table,
+                         double delta, int64_t *coeffs);
+void orc_ckks_decode_f64(const int64_t *coeffs, int n_power, int batch, const double *table,
+                         double delta, double *slots);
+
 /* ---- exact integer products ---- */
 /* linear product, out has 2n entries (last one 0): fft_cpu.cu:52-73 without % */
 void orc_schoolbook_linear(int n, const int64_t *a, const int64_t *b, int64_t *out);

@@ -222,3 +222,22 @@ def test_golden_fixtures_regenerate_identically():
             assert g[k].dtype == fresh[k].dtype and np.array_equal(g[k], fresh[k]), k
         assert np.array_equal(g["c1_product"], O.schoolbook_linear(g["c1_a"], g["c1_b"]))
         assert np.array_equal(g["ffnt_product"], O.schoolbook_negacyclic(g["ffnt_a"], g["ffnt_b"]))
+
+
+@pytest.mark.parametrize("n_power", [1, 2, 5, 8])
+def test_ckks_oracle_is_canonical_embedding(n_power):
+    """The oracle's CKKS decode is the canonical embedding at the roots
+    zeta_2N^(5^j mod 2N) (brute force), and its encode inverts it exactly on
+    integer coefficient vectors."""
+    l, delta = 1 << n_power, 2.0 ** 20
+    N = 2 * l
+    rng = np.random.default_rng(900 + n_power)
+    m = rng.integers(-2 ** 35, 2 ** 35, size=2 * l)
+    z = O.ckks_decode(m, n_power, 1, O.special_table(l), delta)
+    e = np.array([pow(5, j, 2 * N) for j in range(l)])
+    ref = np.exp(2j * np.pi * np.outer(e, np.arange(2 * l)) / (2 * N)) @ m.astype(float) / delta
+    assert np.max(np.abs(z - ref)) <= 1e-12 * np.max(np.abs(ref))
+    assert np.array_equal(O.ckks_encode(ref, n_power, 1, O.special_table(l, True), delta), m)
+    # a planted mistake (natural instead of bit-reversed slot order) is caught
+    zb = ref[[O.bitreverse(i, n_power) for i in range(l)]] if l > 2 else ref[::-1]
+    assert not np.array_equal(O.ckks_encode(zb, n_power, 1, O.special_table(l, True), delta), m)
