@@ -166,42 +166,65 @@ __device__ __forceinline__ void top_row_coeffs(const ProdArgs<T>& a, int q, cplx
     }
 }
 
-template <typename T, int S, int SB, int E, int PL>
+// Gathers the sub-problem's input into LDS: fold + twist + row q of the top
+// network for every position c < 2^S.  With FT, the first Cooley-Tukey stage
+// of the sub-problem (bit S-1, a one-bit step that would otherwise cost a
+// full LDS round trip) is applied on the way: positions c and c + 2^(S-1) are
+// gathered by the same thread, and the stage's twiddle, tab[base + q], is
+// uniform over the workgroup (wtop).
+template <typename T, int S, int SB, int E, int PL, bool FT>
 __device__ __forceinline__ void load_sub(cplx<T>* lds, const ProdArgs<T>& a, rsrc_t x, rsrc_t tw,
                                          int w, int limb, int k, long long o, const cplx<T>* C,
-                                         int tid)
+                                         cplx<T> wtop, int tid)
 {
     constexpr int NT = (1 << S) >> E;
     constexpr int Q = 1 << SB;
     constexpr int V = in_vec<T, PL>();
     constexpr int NG = (1 << E) / V;  // groups of V consecutive positions per thread
-    constexpr int U = GFA_LOAD_UNROLL;  // groups in flight per thread
-    static_assert(NG % U == 0, "load unroll must divide the groups per thread");
+    constexpr int H = FT ? 2 : 1;     // groups combined by the fused stage
+    constexpr int NGH = NG / H;       // partner of group g: g + NGH (c + 2^(S-1))
+    constexpr int U = GFA_LOAD_UNROLL;  // group pairs in flight per thread
+    static_assert(NGH % U == 0, "load unroll must divide the groups per thread");
     const int L = 1 << a.n;
 #pragma unroll 1
-    for (int g0 = 0; g0 < NG; g0 += U) {
-        T re[U][Q][V], im[U][Q][V];
-        cplx<T> tl[U][V];
+    for (int g0 = 0; g0 < NGH; g0 += U) {
+        T re[U][H][Q][V], im[U][H][Q][V];
+        cplx<T> tl[U][H][V];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const int c0 = (tid + (g0 + u) * NT) * V;
-            twist_vec<T, V>(tw, c0, tl[u]);
 #pragma unroll
-            for (int j = 0; j < Q; j++)
-                prod_input_vec<T, PL>(x, L, w, limb, k, o, (j << S) | c0, re[u][j], im[u][j]);
+            for (int h = 0; h < H; h++) {
+                const int c0 = (tid + (g0 + u + h * NGH) * NT) * V;
+                twist_vec<T, V>(tw, c0, tl[u][h]);
+#pragma unroll
+                for (int j = 0; j < Q; j++)
+                    prod_input_vec<T, PL>(x, L, w, limb, k, o, (j << S) | c0, re[u][h][j],
+                                          im[u][h][j]);
+            }
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const int c0 = (tid + (g0 + u) * NT) * V;
 #pragma unroll
             for (int v = 0; v < V; v++) {
-                T ax = T(0), ay = T(0);
+                cplx<T> y[H];
 #pragma unroll
-                for (int j = 0; j < Q; j++) {
-                    ax = ax + C[j].x * re[u][j][v] - C[j].y * im[u][j][v];
-                    ay = ay + C[j].x * im[u][j][v] + C[j].y * re[u][j][v];
+                for (int h = 0; h < H; h++) {
+                    T ax = T(0), ay = T(0);
+#pragma unroll
+                    for (int j = 0; j < Q; j++) {
+                        ax = ax + C[j].x * re[u][h][j][v] - C[j].y * im[u][h][j][v];
+                        ay = ay + C[j].x * im[u][h][j][v] + C[j].y * re[u][h][j][v];
+                    }
+                    y[h] = tl[u][h][v] * mk<T>(ax, ay);
                 }
-                lds[swz(c0 + v)] = tl[u][v] * mk<T>(ax, ay);
+                const int c0 = (tid + (g0 + u) * NT) * V + v;
+                if constexpr (FT) {
+                    const cplx<T> Vw = y[1] * wtop;
+                    lds[swz(c0)] = y[0] + Vw;
+                    lds[swz(c0 + (1 << (S - 1)))] = y[0] - Vw;
+                } else {
+                    lds[swz(c0)] = y[0];
+                }
             }
         }
     }
@@ -323,6 +346,18 @@ __global__ __launch_bounds__((1 << S) >> E) void product_sub_kernel(ProdArgs<T> 
 
     cplx<T> C[Q];
     top_row_coeffs<T, SB>(a, q, C);
+    // A one-bit top step (S = RB m + 1) is fused into the gather (forward)
+    // and into the final store (inverse); its twiddles tab[base + q],
+    // itab[base + q] are uniform (position q 2^S + c, bit S-1).
+    constexpr bool FT = (S % RB) == 1 && S >= 2;
+    cplx<T> wtop = mk<T>(T(1), T(0)), witop = wtop;
+    if constexpr (FT) {
+        const int tbase = a.nega ? (1 << (a.n - S)) : 0;
+        wtop = a.tab[tbase + q];
+        witop = a.itab[tbase + q];
+    }
+    constexpr int FS0 = FT ? 1 : 0;               // first forward step through LDS
+    constexpr int IEND = FT ? NST - 1 : NST;      // end of the inverse steps through LDS
 
     cplx<T> breg[EPT];
     for (int jb = 0; jb < KB; jb++) {
@@ -331,15 +366,15 @@ __global__ __launch_bounds__((1 << S) >> E) void product_sub_kernel(ProdArgs<T> 
         int t = tid;
         asm volatile("" : "+v"(t));
         if (jb) __syncthreads();
-        load_sub<T, S, SB, E, PL>(lds, a, xb, tw, a.wb, jb, KB, a.ob, C, t);
-        run_steps_ct<T, MODE_STD_FWD, E, NT, S, RB, 0, NST - 1, true>(lds, pf, g, t, a.wc);
+        load_sub<T, S, SB, E, PL, FT>(lds, a, xb, tw, a.wb, jb, KB, a.ob, C, wtop, t);
+        run_steps_ct<T, MODE_STD_FWD, E, NT, S, RB, FS0, NST - 1, true>(lds, pf, g, t, a.wc);
         last_forward_step<T, E, RB, NT>(lds, pf, g, t, breg, a.wc);
         for (int ia = 0; ia < KA; ia++) {
             int u = tid;
             asm volatile("" : "+v"(u));
             __syncthreads();
-            load_sub<T, S, SB, E, PL>(lds, a, xa, tw, a.wa, ia, KA, a.oa, C, u);
-            run_steps_ct<T, MODE_STD_FWD, E, NT, S, RB, 0, NST - 1, true>(lds, pf, g, u, a.wc);
+            load_sub<T, S, SB, E, PL, FT>(lds, a, xa, tw, a.wa, ia, KA, a.oa, C, wtop, u);
+            run_steps_ct<T, MODE_STD_FWD, E, NT, S, RB, FS0, NST - 1, true>(lds, pf, g, u, a.wc);
             // last forward step, product, first inverse step: in registers,
             // one radix-2^RB group at a time
 #pragma unroll
@@ -355,24 +390,44 @@ __global__ __launch_bounds__((1 << S) >> E) void product_sub_kernel(ProdArgs<T> 
 #pragma unroll
                 for (int j = 0; j < (1 << RB); j++) lds[swz(rbase | j)] = v[j];
             }
-            run_steps_ct<T, MODE_STD_INV, E, NT, S, RB, 1, NST, true>(lds, pinv, gi, u, a.wci);
+            run_steps_ct<T, MODE_STD_INV, E, NT, S, RB, 1, IEND, true>(lds, pinv, gi, u, a.wci);
+            // element i of this thread: position u + i NT; with FT the last
+            // Gentleman-Sande stage pairs elements i and i + EPT/2
+            constexpr int NI = FT ? EPT / 2 : EPT;
             if constexpr (SB == 0 && PL == PL_FOLD) {
                 // whole product in this workgroup: scale, untwist, unfold
                 T* o = a.out + p * a.out_stride;
                 const rsrc_t utw = make_rsrc(a.untwist);
 #pragma unroll 4
-                for (int i = 0; i < EPT; i++) {
+                for (int i = 0; i < NI; i++) {
                     const int c = u + i * NT;
-                    cplx<T> z = (lds[swz(c)] * a.scale) * bldc<T>(utw, c);
-                    o[c] = z.x;
-                    o[c + L] = z.y;
+                    cplx<T> z0 = lds[swz(c)];
+                    if constexpr (FT) {
+                        const int c1 = c + (1 << (S - 1));
+                        const cplx<T> z1 = lds[swz(c1)];
+                        const cplx<T> d = (z0 - z1) * witop;
+                        z0 = z0 + z1;
+                        const cplx<T> r1 = (d * a.scale) * bldc<T>(utw, c1);
+                        o[c1] = r1.x;
+                        o[c1 + L] = r1.y;
+                    }
+                    const cplx<T> r0 = (z0 * a.scale) * bldc<T>(utw, c);
+                    o[c] = r0.x;
+                    o[c + L] = r0.y;
                 }
             } else {
                 cplx<T>* dst = a.scratch + (p * nprod + ia * KB + jb) * L + ((long long)q << S);
 #pragma unroll 4
-                for (int i = 0; i < EPT; i++) {
+                for (int i = 0; i < NI; i++) {
                     const int c = u + i * NT;
-                    stc(dst + c, lds[swz(c)]);
+                    cplx<T> z0 = lds[swz(c)];
+                    if constexpr (FT) {
+                        const int c1 = c + (1 << (S - 1));
+                        const cplx<T> z1 = lds[swz(c1)];
+                        stc(dst + c1, (z0 - z1) * witop);
+                        z0 = z0 + z1;
+                    }
+                    stc(dst + c, z0);
                 }
             }
         }
