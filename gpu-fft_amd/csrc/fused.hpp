@@ -103,20 +103,6 @@ __device__ __forceinline__ void prod_input_vec(rsrc_t x, int L, int w, int limb,
     }
 }
 
-// V consecutive twist factors tw[c0 .. c0+V-1] (c0 a multiple of V), in
-// 16-byte loads (one complex double, or two complex floats, per load).
-template <typename T, int V>
-__device__ __forceinline__ void twist_vec(rsrc_t tw, int c0, cplx<T>* t)
-{
-    constexpr int PER = 16 / (int)sizeof(cplx<T>);
-    static_assert(V % PER == 0, "whole 16-byte twist loads");
-#pragma unroll
-    for (int v = 0; v < V; v += PER) {
-        const u32x4 r = __builtin_amdgcn_raw_buffer_load_b128(tw, (unsigned)(c0 + v) * (unsigned)sizeof(cplx<T>), 0, 0);
-        __builtin_memcpy(t + v, &r, 16);
-    }
-}
-
 // Row q of the top SB forward stages (bits n-1 .. S) of zz[j] = input at
 // position j 2^S + c.  Only the entries whose processed bits equal q's
 // survive each stage, and their twiddle, tab[base + (q >> (i+1))], is
@@ -172,10 +158,35 @@ __device__ __forceinline__ void top_row_coeffs(const ProdArgs<T>& a, int q, cplx
 // full LDS round trip) is applied on the way: positions c and c + 2^(S-1) are
 // gathered by the same thread, and the stage's twiddle, tab[base + q], is
 // uniform over the workgroup (wtop).
+// Twist factors twist[c], c < 2^S, as twist[c & ~(2^TWB-1)] * twist[c & (2^TWB-1)]
+// (exp(2 pi i c / 2N) is multiplicative in c; both factors are entries of the
+// FFNT twist table), from two small LDS tables: the gather then moves only
+// operand bytes through the vector memory path.
+template <int S> struct TwistSplit {
+    static constexpr int TWB = S > 7 ? 7 : S;
+    static constexpr int NLO = 1 << TWB;
+    static constexpr int NHI = 1 << (S - TWB);
+};
+
+template <typename T, int S>
+__device__ __forceinline__ void fill_twist_split(cplx<T>* tws, const cplx<T>* twist, int tid, int nthreads)
+{
+    using TS = TwistSplit<S>;
+    for (int i = tid; i < TS::NHI + TS::NLO; i += nthreads)
+        tws[i] = i < TS::NHI ? twist[i << TS::TWB] : twist[i - TS::NHI];
+}
+
+template <typename T, int S>
+__device__ __forceinline__ cplx<T> twist_of(const cplx<T>* tws, int c)
+{
+    using TS = TwistSplit<S>;
+    return tws[c >> TS::TWB] * tws[TS::NHI + (c & (TS::NLO - 1))];
+}
+
 template <typename T, int S, int SB, int E, int PL, bool FT>
-__device__ __forceinline__ void load_sub(cplx<T>* lds, const ProdArgs<T>& a, rsrc_t x, rsrc_t tw,
-                                         int w, int limb, int k, long long o, const cplx<T>* C,
-                                         cplx<T> wtop, int tid)
+__device__ __forceinline__ void load_sub(cplx<T>* lds, const ProdArgs<T>& a, rsrc_t x,
+                                         const cplx<T>* tws, int w, int limb, int k, long long o,
+                                         const cplx<T>* C, cplx<T> wtop, int tid)
 {
     constexpr int NT = (1 << S) >> E;
     constexpr int Q = 1 << SB;
@@ -189,13 +200,11 @@ __device__ __forceinline__ void load_sub(cplx<T>* lds, const ProdArgs<T>& a, rsr
 #pragma unroll 1
     for (int g0 = 0; g0 < NGH; g0 += U) {
         T re[U][H][Q][V], im[U][H][Q][V];
-        cplx<T> tl[U][H][V];
 #pragma unroll
         for (int u = 0; u < U; u++) {
 #pragma unroll
             for (int h = 0; h < H; h++) {
                 const int c0 = (tid + (g0 + u + h * NGH) * NT) * V;
-                twist_vec<T, V>(tw, c0, tl[u][h]);
 #pragma unroll
                 for (int j = 0; j < Q; j++)
                     prod_input_vec<T, PL>(x, L, w, limb, k, o, (j << S) | c0, re[u][h][j],
@@ -215,7 +224,8 @@ __device__ __forceinline__ void load_sub(cplx<T>* lds, const ProdArgs<T>& a, rsr
                         ax = ax + C[j].x * re[u][h][j][v] - C[j].y * im[u][h][j][v];
                         ay = ay + C[j].x * im[u][h][j][v] + C[j].y * re[u][h][j][v];
                     }
-                    y[h] = tl[u][h][v] * mk<T>(ax, ay);
+                    y[h] = twist_of<T, S>(tws, (tid + (g0 + u + h * NGH) * NT) * V + v) *
+                           mk<T>(ax, ay);
                 }
                 const int c0 = (tid + (g0 + u) * NT) * V + v;
                 if constexpr (FT) {
@@ -304,6 +314,7 @@ __global__ __launch_bounds__((1 << S) >> E) void product_sub_kernel(ProdArgs<T> 
     constexpr int NT = TILE / EPT;
     constexpr int Q = 1 << SB;
     __shared__ cplx<T> lds[TILE];
+    __shared__ cplx<T> tws[TwistSplit<S>::NHI + TwistSplit<S>::NLO];
 
     const int tid = threadIdx.x;
     // XCD-aware order: consecutive workgroups go to consecutive XCDs, so the
@@ -339,7 +350,7 @@ __global__ __launch_bounds__((1 << S) >> E) void product_sub_kernel(ProdArgs<T> 
                                 p * a.in_stride * (PL == PL_FOLD ? (long long)sizeof(T) : 4LL));
     const rsrc_t xb = make_rsrc(reinterpret_cast<const char*>(a.b) +
                                 p * a.in_stride * (PL == PL_FOLD ? (long long)sizeof(T) : 4LL));
-    const rsrc_t tw = make_rsrc(a.twist);
+    fill_twist_split<T, S>(tws, a.twist, tid, NT);  // visible after the first barrier below
     // compile-time single limb for the FP64 form
     const int KA = PL == PL_FOLD ? 1 : a.ka;
     const int KB = PL == PL_FOLD ? 1 : a.kb;
@@ -365,15 +376,15 @@ __global__ __launch_bounds__((1 << S) >> E) void product_sub_kernel(ProdArgs<T> 
         // the loop body inside the loop (hoisted, it would hold registers)
         int t = tid;
         asm volatile("" : "+v"(t));
-        if (jb) __syncthreads();
-        load_sub<T, S, SB, E, PL, FT>(lds, a, xb, tw, a.wb, jb, KB, a.ob, C, wtop, t);
+        __syncthreads();
+        load_sub<T, S, SB, E, PL, FT>(lds, a, xb, tws, a.wb, jb, KB, a.ob, C, wtop, t);
         run_steps_ct<T, MODE_STD_FWD, E, NT, S, RB, FS0, NST - 1, true>(lds, pf, g, t, a.wc);
         last_forward_step<T, E, RB, NT>(lds, pf, g, t, breg, a.wc);
         for (int ia = 0; ia < KA; ia++) {
             int u = tid;
             asm volatile("" : "+v"(u));
             __syncthreads();
-            load_sub<T, S, SB, E, PL, FT>(lds, a, xa, tw, a.wa, ia, KA, a.oa, C, wtop, u);
+            load_sub<T, S, SB, E, PL, FT>(lds, a, xa, tws, a.wa, ia, KA, a.oa, C, wtop, u);
             run_steps_ct<T, MODE_STD_FWD, E, NT, S, RB, FS0, NST - 1, true>(lds, pf, g, u, a.wc);
             // last forward step, product, first inverse step: in registers,
             // one radix-2^RB group at a time
