@@ -468,6 +468,77 @@ __global__ __launch_bounds__((1 << S) >> E) void product_sub_kernel(ProdArgs<T> 
 // One thread per (product, sub-problem position c).  err (optional) receives
 // per product the largest |x - rint(x)| (as double bits) and the sum of
 // squares over all limb outputs.
+// FP64 products: top SB Gentleman-Sande stages (bits S .. n-1), scale,
+// untwist and unfold (reference InverseFFNTCoreUnique, src/lib/fft.cu:
+// 1298-1384), one thread per pair of adjacent columns c, c+1 of one product:
+// 16-byte loads of the 2^SB rows, 16-byte stores of the two real and the two
+// imaginary outputs, untwist[P] = untwist[P & ~127] * untwist[P & 127] from
+// LDS (the table is exp(-2 pi i P / 2N), multiplicative in P).
+struct F64CombineArgs {
+    int n, S, batch;
+    const cplx<double>* scratch;  // [batch][2^n]
+    const cplx<double>* itab;
+    const cplx<double>* untwist;
+    double scale;
+    double* out;                  // [batch][out_stride]
+    long long out_stride;
+};
+
+constexpr int F64C_MAX_N = 16;  // 2^(16-7) entries in the LDS high-part table
+
+template <int SB>
+__global__ __launch_bounds__(256) void f64_combine_kernel(F64CombineArgs a)
+{
+    constexpr int Q = 1 << SB;
+    __shared__ cplx<double> ut[128 + (1 << (F64C_MAX_N - 7))];
+    const int tid = threadIdx.x;
+    const int nhi = 1 << (a.n - 7);
+    for (int i = tid; i < 128 + nhi; i += 256)
+        ut[i] = i < 128 ? a.untwist[i] : a.untwist[(i - 128) << 7];
+    __syncthreads();
+    const long long gid = (long long)blockIdx.x * 256 + tid;
+    const long long p = gid >> (a.S - 1);
+    const int c = (int)(gid & ((1LL << (a.S - 1)) - 1)) * 2;
+    if (p >= a.batch) return;
+    const long long L = 1LL << a.n;
+    const cplx<double>* src = a.scratch + p * L;
+    cplx<double> w[Q > 1 ? Q / 2 : 1];
+#pragma unroll
+    for (int m = 0; m < Q / 2; m++) w[m] = a.itab[m];
+    cplx<double> y[2][Q];
+#pragma unroll
+    for (int j = 0; j < Q; j++) {
+        y[0][j] = ldc(src + ((long long)j << a.S) + c);
+        y[1][j] = ldc(src + ((long long)j << a.S) + c + 1);
+    }
+    double* o = a.out + p * a.out_stride;
+#pragma unroll
+    for (int v = 0; v < 2; v++) {
+#pragma unroll
+        for (int i = 0; i < SB; i++) {
+#pragma unroll
+            for (int j0 = 0; j0 < Q; j0++) {
+                if (j0 & (1 << i)) continue;
+                const int j1 = j0 | (1 << i);
+                const cplx<double> U = y[v][j0], V = y[v][j1];
+                y[v][j0] = U + V;
+                y[v][j1] = (U - V) * w[j0 >> (i + 1)];
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < Q; j++) {
+        const int P = (j << a.S) | c;
+        cplx<double> z[2];
+#pragma unroll
+        for (int v = 0; v < 2; v++)
+            z[v] = mk<double>(y[v][j].x * a.scale, y[v][j].y * a.scale) *
+                   (ut[128 + ((P + v) >> 7)] * ut[(P + v) & 127]);
+        stc(reinterpret_cast<cplx<double>*>(o + P), mk<double>(z[0].x, z[1].x));
+        stc(reinterpret_cast<cplx<double>*>(o + P + L), mk<double>(z[0].y, z[1].y));
+    }
+}
+
 template <int SB> struct CombineArgs {
     int n, S, batch, nprod, kb, wa, wb;
     const cplx<float>* scratch;

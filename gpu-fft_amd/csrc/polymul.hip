@@ -315,25 +315,23 @@ int gpufft_polymul_f64_phases(gpufft_polymul_plan plan, const double* a, const d
             if (int s = launch_products<double, PL_FOLD>(plan, pa, pb, po, nb, cmb, st, who)) return s;
         if (plan->SB > 0 && !cmb && (phases & GPUFFT_PHASE_COMBINE)) {
             // top SB inverse stages + scale + untwist + unfold
-            PassArgs<double> pa2{};
-            pa2.n = plan->n;
-            pa2.blo = plan->S;
-            pa2.K = plan->SB;
-            pa2.logC = 12 - plan->SB;
-            pa2.logBT = 0;
-            pa2.batch = nb;
-            pa2.nega = 0;
-            pa2.do_scale = 1;
-            pa2.scale = mk<double>(1.0 / (double)(1LL << plan->n), 0.0);
-            pa2.tab = (const cplx<double>*)plan->iroot;
-            pa2.src = plan->scratch;
-            pa2.src_stride = 1LL << plan->n;
-            pa2.dst = po;
-            pa2.stab = (const cplx<double>*)plan->untwist;
-            pa2.dst_stride = N;
-            const long long tiles = ((long long)nb << plan->n) >> 12;
-            hipLaunchKernelGGL((pass_kernel<double, MODE_STD_INV, 12, 4, LD_CPLX, ST_UNFOLD>),
-                               dim3((unsigned)tiles), dim3(256), 0, st, pa2);
+            F64CombineArgs ca{};
+            ca.n = plan->n;
+            ca.S = plan->S;
+            ca.batch = nb;
+            ca.scratch = (const cplx<double>*)plan->scratch;
+            ca.itab = (const cplx<double>*)plan->iroot;
+            ca.untwist = (const cplx<double>*)plan->untwist;
+            ca.scale = 1.0 / (double)(1LL << plan->n);
+            ca.out = po;
+            ca.out_stride = N;
+            const unsigned blocks = (unsigned)((((long long)nb << (plan->S - 1)) + 255) / 256);
+            switch (plan->SB) {
+                case 1: hipLaunchKernelGGL(f64_combine_kernel<1>, dim3(blocks), dim3(256), 0, st, ca); break;
+                case 2: hipLaunchKernelGGL(f64_combine_kernel<2>, dim3(blocks), dim3(256), 0, st, ca); break;
+                case 3: hipLaunchKernelGGL(f64_combine_kernel<3>, dim3(blocks), dim3(256), 0, st, ca); break;
+                default: return fail(GPUFFT_ERR_UNSUPPORTED, std::string(who) + ": split too deep");
+            }
             GFA_CHECK_LAUNCH(who);
         }
     }
