@@ -152,12 +152,6 @@ __device__ __forceinline__ void top_row_coeffs(const ProdArgs<T>& a, int q, cplx
     }
 }
 
-// Gathers the sub-problem's input into LDS: fold + twist + row q of the top
-// network for every position c < 2^S.  With FT, the first Cooley-Tukey stage
-// of the sub-problem (bit S-1, a one-bit step that would otherwise cost a
-// full LDS round trip) is applied on the way: positions c and c + 2^(S-1) are
-// gathered by the same thread, and the stage's twiddle, tab[base + q], is
-// uniform over the workgroup (wtop).
 // Twist factors twist[c], c < 2^S, as twist[c & ~(2^TWB-1)] * twist[c & (2^TWB-1)]
 // (exp(2 pi i c / 2N) is multiplicative in c; both factors are entries of the
 // FFNT twist table), from two small LDS tables: the gather then moves only
@@ -183,59 +177,96 @@ __device__ __forceinline__ cplx<T> twist_of(const cplx<T>* tws, int c)
     return tws[c >> TS::TWB] * tws[TS::NHI + (c & (TS::NLO - 1))];
 }
 
-template <typename T, int S, int SB, int E, int PL, bool FT>
+// The short top step of a sub-problem (its HB = S mod RB highest bits, a
+// step that would otherwise cost a full LDS round trip) runs in registers:
+// forward in the gather, inverse in the final store.  A thread holds the
+// 2^HB positions c + m 2^(S-HB), m < 2^HB, bit k of m being position bit
+// S-HB+k, and the stage twiddles tab[base_b + ((q 2^S + c) >> (b+1))] =
+// tab[base_b + (q << (HB-1-k)) + (m >> (k+1))] (b = S-HB+k) are uniform over
+// the workgroup (scalar loads).
+template <typename T, int S, int HB>
+__device__ __forceinline__ cplx<T> top_twiddle(const ProdArgs<T>& a, const cplx<T>* tab, int q,
+                                               int k, int t)
+{
+    const int b = S - HB + k;
+    const int base = a.nega ? (1 << (a.n - 1 - b)) : 0;
+    return tab[base + (q << (HB - 1 - k)) + t];
+}
+
+template <typename T, int S, int HB>
+__device__ __forceinline__ void top_ct(cplx<T>* y, const ProdArgs<T>& a, int q)
+{
+#pragma unroll
+    for (int k = HB - 1; k >= 0; k--) {
+#pragma unroll
+        for (int m = 0; m < (1 << HB); m++) {
+            if (m & (1 << k)) continue;
+            const cplx<T> w = top_twiddle<T, S, HB>(a, a.tab, q, k, m >> (k + 1));
+            const cplx<T> U = y[m], Vw = y[m | (1 << k)] * w;
+            y[m] = U + Vw;
+            y[m | (1 << k)] = U - Vw;
+        }
+    }
+}
+
+template <typename T, int S, int HB>
+__device__ __forceinline__ void top_gs(cplx<T>* y, const ProdArgs<T>& a, int q)
+{
+#pragma unroll
+    for (int k = 0; k < HB; k++) {
+#pragma unroll
+        for (int m = 0; m < (1 << HB); m++) {
+            if (m & (1 << k)) continue;
+            const cplx<T> w = top_twiddle<T, S, HB>(a, a.itab, q, k, m >> (k + 1));
+            const cplx<T> U = y[m], V = y[m | (1 << k)];
+            y[m] = U + V;
+            y[m | (1 << k)] = (U - V) * w;
+        }
+    }
+}
+
+// Gathers the sub-problem's input into LDS: fold + twist + row q of the top
+// network for every position c < 2^S, then the HB-bit top step (top_ct).
+template <typename T, int S, int SB, int E, int PL, int HB>
 __device__ __forceinline__ void load_sub(cplx<T>* lds, const ProdArgs<T>& a, rsrc_t x,
                                          const cplx<T>* tws, int w, int limb, int k, long long o,
-                                         const cplx<T>* C, cplx<T> wtop, int tid)
+                                         const cplx<T>* C, int q, int tid)
 {
     constexpr int NT = (1 << S) >> E;
     constexpr int Q = 1 << SB;
     constexpr int V = in_vec<T, PL>();
     constexpr int NG = (1 << E) / V;  // groups of V consecutive positions per thread
-    constexpr int H = FT ? 2 : 1;     // groups combined by the fused stage
-    constexpr int NGH = NG / H;       // partner of group g: g + NGH (c + 2^(S-1))
-    constexpr int U = GFA_LOAD_UNROLL;  // group pairs in flight per thread
-    static_assert(NGH % U == 0, "load unroll must divide the groups per thread");
+    constexpr int NP = 1 << HB;       // groups joined by the top step
+    constexpr int NGH = NG / NP;      // partner m of group g: g + m NGH (c + m 2^(S-HB))
+    static_assert(NGH >= 1, "the top step must fit the thread's groups");
     const int L = 1 << a.n;
 #pragma unroll 1
-    for (int g0 = 0; g0 < NGH; g0 += U) {
-        T re[U][H][Q][V], im[U][H][Q][V];
+    for (int g0 = 0; g0 < NGH; g0++) {
+        cplx<T> y[V][NP];
 #pragma unroll
-        for (int u = 0; u < U; u++) {
+        for (int m = 0; m < NP; m++) {
+            const int c0 = (tid + (g0 + m * NGH) * NT) * V;
+            T re[Q][V], im[Q][V];
 #pragma unroll
-            for (int h = 0; h < H; h++) {
-                const int c0 = (tid + (g0 + u + h * NGH) * NT) * V;
+            for (int j = 0; j < Q; j++)
+                prod_input_vec<T, PL>(x, L, w, limb, k, o, (j << S) | c0, re[j], im[j]);
 #pragma unroll
-                for (int j = 0; j < Q; j++)
-                    prod_input_vec<T, PL>(x, L, w, limb, k, o, (j << S) | c0, re[u][h][j],
-                                          im[u][h][j]);
+            for (int v = 0; v < V; v++) {
+                T ax = T(0), ay = T(0);
+#pragma unroll
+                for (int j = 0; j < Q; j++) {
+                    ax = ax + C[j].x * re[j][v] - C[j].y * im[j][v];
+                    ay = ay + C[j].x * im[j][v] + C[j].y * re[j][v];
+                }
+                y[v][m] = twist_of<T, S>(tws, c0 + v) * mk<T>(ax, ay);
             }
         }
 #pragma unroll
-        for (int u = 0; u < U; u++) {
+        for (int v = 0; v < V; v++) {
+            if constexpr (HB > 0) top_ct<T, S, HB>(y[v], a, q);
 #pragma unroll
-            for (int v = 0; v < V; v++) {
-                cplx<T> y[H];
-#pragma unroll
-                for (int h = 0; h < H; h++) {
-                    T ax = T(0), ay = T(0);
-#pragma unroll
-                    for (int j = 0; j < Q; j++) {
-                        ax = ax + C[j].x * re[u][h][j][v] - C[j].y * im[u][h][j][v];
-                        ay = ay + C[j].x * im[u][h][j][v] + C[j].y * re[u][h][j][v];
-                    }
-                    y[h] = twist_of<T, S>(tws, (tid + (g0 + u + h * NGH) * NT) * V + v) *
-                           mk<T>(ax, ay);
-                }
-                const int c0 = (tid + (g0 + u) * NT) * V + v;
-                if constexpr (FT) {
-                    const cplx<T> Vw = y[1] * wtop;
-                    lds[swz(c0)] = y[0] + Vw;
-                    lds[swz(c0 + (1 << (S - 1)))] = y[0] - Vw;
-                } else {
-                    lds[swz(c0)] = y[0];
-                }
-            }
+            for (int m = 0; m < NP; m++)
+                lds[swz((tid + (g0 + m * NGH) * NT) * V + v)] = y[v][m];
         }
     }
 }
@@ -357,18 +388,11 @@ __global__ __launch_bounds__((1 << S) >> E) void product_sub_kernel(ProdArgs<T> 
 
     cplx<T> C[Q];
     top_row_coeffs<T, SB>(a, q, C);
-    // A one-bit top step (S = RB m + 1) is fused into the gather (forward)
-    // and into the final store (inverse); its twiddles tab[base + q],
-    // itab[base + q] are uniform (position q 2^S + c, bit S-1).
-    constexpr bool FT = (S % RB) == 1 && S >= 2;
-    cplx<T> wtop = mk<T>(T(1), T(0)), witop = wtop;
-    if constexpr (FT) {
-        const int tbase = a.nega ? (1 << (a.n - S)) : 0;
-        wtop = a.tab[tbase + q];
-        witop = a.itab[tbase + q];
-    }
-    constexpr int FS0 = FT ? 1 : 0;               // first forward step through LDS
-    constexpr int IEND = FT ? NST - 1 : NST;      // end of the inverse steps through LDS
+    // the short top step (HB bits) runs in registers: in the gather (forward)
+    // and in the final store (inverse)
+    constexpr int HB = S % RB;
+    constexpr int FS0 = HB ? 1 : 0;           // first forward step through LDS
+    constexpr int IEND = HB ? NST - 1 : NST;  // end of the inverse steps through LDS
 
     cplx<T> breg[EPT];
     for (int jb = 0; jb < KB; jb++) {
@@ -377,14 +401,14 @@ __global__ __launch_bounds__((1 << S) >> E) void product_sub_kernel(ProdArgs<T> 
         int t = tid;
         asm volatile("" : "+v"(t));
         __syncthreads();
-        load_sub<T, S, SB, E, PL, FT>(lds, a, xb, tws, a.wb, jb, KB, a.ob, C, wtop, t);
+        load_sub<T, S, SB, E, PL, HB>(lds, a, xb, tws, a.wb, jb, KB, a.ob, C, q, t);
         run_steps_ct<T, MODE_STD_FWD, E, NT, S, RB, FS0, NST - 1, true>(lds, pf, g, t, a.wc);
         last_forward_step<T, E, RB, NT>(lds, pf, g, t, breg, a.wc);
         for (int ia = 0; ia < KA; ia++) {
             int u = tid;
             asm volatile("" : "+v"(u));
             __syncthreads();
-            load_sub<T, S, SB, E, PL, FT>(lds, a, xa, tws, a.wa, ia, KA, a.oa, C, wtop, u);
+            load_sub<T, S, SB, E, PL, HB>(lds, a, xa, tws, a.wa, ia, KA, a.oa, C, q, u);
             run_steps_ct<T, MODE_STD_FWD, E, NT, S, RB, FS0, NST - 1, true>(lds, pf, g, u, a.wc);
             // last forward step, product, first inverse step: in registers,
             // one radix-2^RB group at a time
@@ -402,43 +426,40 @@ __global__ __launch_bounds__((1 << S) >> E) void product_sub_kernel(ProdArgs<T> 
                 for (int j = 0; j < (1 << RB); j++) lds[swz(rbase | j)] = v[j];
             }
             run_steps_ct<T, MODE_STD_INV, E, NT, S, RB, 1, IEND, true>(lds, pinv, gi, u, a.wci);
-            // element i of this thread: position u + i NT; with FT the last
-            // Gentleman-Sande stage pairs elements i and i + EPT/2
-            constexpr int NI = FT ? EPT / 2 : EPT;
+            // element i of this thread: position u + i NT; the top step joins
+            // elements i + m EPT/2^HB (positions c + m 2^(S-HB))
+            constexpr int NP = 1 << HB;
+            constexpr int NI = EPT / NP;
             if constexpr (SB == 0 && PL == PL_FOLD) {
                 // whole product in this workgroup: scale, untwist, unfold
                 T* o = a.out + p * a.out_stride;
                 const rsrc_t utw = make_rsrc(a.untwist);
-#pragma unroll 4
+#pragma unroll 2
                 for (int i = 0; i < NI; i++) {
                     const int c = u + i * NT;
-                    cplx<T> z0 = lds[swz(c)];
-                    if constexpr (FT) {
-                        const int c1 = c + (1 << (S - 1));
-                        const cplx<T> z1 = lds[swz(c1)];
-                        const cplx<T> d = (z0 - z1) * witop;
-                        z0 = z0 + z1;
-                        const cplx<T> r1 = (d * a.scale) * bldc<T>(utw, c1);
-                        o[c1] = r1.x;
-                        o[c1 + L] = r1.y;
+                    cplx<T> z[NP];
+#pragma unroll
+                    for (int m = 0; m < NP; m++) z[m] = lds[swz(c + (m << (S - HB)))];
+                    if constexpr (HB > 0) top_gs<T, S, HB>(z, a, q);
+#pragma unroll
+                    for (int m = 0; m < NP; m++) {
+                        const int cm = c + (m << (S - HB));
+                        const cplx<T> r = (z[m] * a.scale) * bldc<T>(utw, cm);
+                        o[cm] = r.x;
+                        o[cm + L] = r.y;
                     }
-                    const cplx<T> r0 = (z0 * a.scale) * bldc<T>(utw, c);
-                    o[c] = r0.x;
-                    o[c + L] = r0.y;
                 }
             } else {
                 cplx<T>* dst = a.scratch + (p * nprod + ia * KB + jb) * L + ((long long)q << S);
-#pragma unroll 4
+#pragma unroll 2
                 for (int i = 0; i < NI; i++) {
                     const int c = u + i * NT;
-                    cplx<T> z0 = lds[swz(c)];
-                    if constexpr (FT) {
-                        const int c1 = c + (1 << (S - 1));
-                        const cplx<T> z1 = lds[swz(c1)];
-                        stc(dst + c1, (z0 - z1) * witop);
-                        z0 = z0 + z1;
-                    }
-                    stc(dst + c, z0);
+                    cplx<T> z[NP];
+#pragma unroll
+                    for (int m = 0; m < NP; m++) z[m] = lds[swz(c + (m << (S - HB)))];
+                    if constexpr (HB > 0) top_gs<T, S, HB>(z, a, q);
+#pragma unroll
+                    for (int m = 0; m < NP; m++) stc(dst + c + (m << (S - HB)), z[m]);
                 }
             }
         }
