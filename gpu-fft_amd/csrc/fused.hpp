@@ -72,26 +72,37 @@ __device__ __forceinline__ int limb_digit(int v, int w, int i, int k, long long 
     return (int)((u >> (w * i)) & ((1LL << w) - 1)) - (1 << (w - 1));
 }
 
-// Positions per load of an operand: 16-byte loads of 2 doubles (FP64 form),
-// 8-byte loads of 2 int32 (limb form; the limb kernels have no registers
-// to spare for wider groups).
-template <typename T, int PL> constexpr int in_vec() { return 2; }
+// Positions per load of an operand: 16-byte loads of 2 doubles (FP64 form);
+// 16-byte loads of 4 int32 (limb form) when the thread's 2^(E-2) groups of
+// 4 still hold the 2^HB partners of the top step, else 8-byte loads of 2.
+template <typename T, int PL, int E, int HB> constexpr int in_vec()
+{
+    return PL == PL_FOLD ? 2 : ((E - 2) >= HB ? 4 : 2);
+}
 
-// Folded inputs of V = in_vec consecutive positions P .. P+V-1 of this
-// product's operand, as T: re[v] = x_{P+v}, im[v] = x_{P+v+L}; two vector
-// buffer loads (P and L are multiples of V).  Wide lanes cut the address and
-// tag work per byte, which bounds the gather of the top network (each input
-// is read by every workgroup of its product).
-template <typename T, int PL>
+// Folded inputs of V consecutive positions P .. P+V-1 of this product's
+// operand, as T: re[v] = x_{P+v}, im[v] = x_{P+v+L}; two vector buffer loads
+// (P and L are multiples of V).  Wide lanes cut the address and tag work per
+// byte, which bounds the gather of the top network (each input is read by
+// every workgroup of its product, each limb's digits by every limb pass).
+template <typename T, int PL, int V>
 __device__ __forceinline__ void prod_input_vec(rsrc_t x, int L, int w, int limb, int k, long long o,
                                                int P, T* re, T* im)
 {
     if constexpr (PL == PL_FOLD) {
-        static_assert(sizeof(T) == 8, "the FP64 form reads doubles");
+        static_assert(sizeof(T) == 8 && V == 2, "the FP64 form reads doubles in pairs");
         const u32x4 r = __builtin_amdgcn_raw_buffer_load_b128(x, (unsigned)P * 8u, 0, 0);
         const u32x4 i = __builtin_amdgcn_raw_buffer_load_b128(x, (unsigned)(P + L) * 8u, 0, 0);
         __builtin_memcpy(re, &r, 16);
         __builtin_memcpy(im, &i, 16);
+    } else if constexpr (V == 4) {
+        const u32x4 r = __builtin_amdgcn_raw_buffer_load_b128(x, (unsigned)P * 4u, 0, 0);
+        const u32x4 i = __builtin_amdgcn_raw_buffer_load_b128(x, (unsigned)(P + L) * 4u, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            re[v] = (T)limb_digit((int)r[v], w, limb, k, o);
+            im[v] = (T)limb_digit((int)i[v], w, limb, k, o);
+        }
     } else {
         const u32x2 r = __builtin_amdgcn_raw_buffer_load_b64(x, (unsigned)P * 4u, 0, 0);
         const u32x2 i = __builtin_amdgcn_raw_buffer_load_b64(x, (unsigned)(P + L) * 4u, 0, 0);
@@ -234,7 +245,7 @@ __device__ __forceinline__ void load_sub(cplx<T>* lds, const ProdArgs<T>& a, rsr
 {
     constexpr int NT = (1 << S) >> E;
     constexpr int Q = 1 << SB;
-    constexpr int V = in_vec<T, PL>();
+    constexpr int V = in_vec<T, PL, E, HB>();
     constexpr int NG = (1 << E) / V;  // groups of V consecutive positions per thread
     constexpr int NP = 1 << HB;       // groups joined by the top step
     constexpr int NGH = NG / NP;      // partner m of group g: g + m NGH (c + m 2^(S-HB))
@@ -249,7 +260,7 @@ __device__ __forceinline__ void load_sub(cplx<T>* lds, const ProdArgs<T>& a, rsr
             T re[Q][V], im[Q][V];
 #pragma unroll
             for (int j = 0; j < Q; j++)
-                prod_input_vec<T, PL>(x, L, w, limb, k, o, (j << S) | c0, re[j], im[j]);
+                prod_input_vec<T, PL, V>(x, L, w, limb, k, o, (j << S) | c0, re[j], im[j]);
 #pragma unroll
             for (int v = 0; v < V; v++) {
                 T ax = T(0), ay = T(0);
