@@ -5,7 +5,8 @@ b uniform in [-2^7, 2^7], seeded splitmix64), inputs already resident in HBM.
 Forms timed, all on one stream with HIP events:
   f64         fused FP64 product (double in / double out), gpufft_polymul_f64
   f32split    fused FP32 limb-split product (int32 in / int64 out),
-              gpufft_polymul_i32, a split into 6 balanced 4-bit limbs
+              gpufft_polymul_i32, a - 2^19 split into 5 balanced 4-bit limbs
+              plus the exact correction limb 2^19 (J b), J = all ones
   f64_dropin  the reference's two-call flow: GPU_FFNT FORWARD on 2*batch
               polynomials, then GPU_FFNT INVERSE with multiplication
 `value` is the poly-mults/s of the fastest form whose rounded products are
@@ -36,7 +37,9 @@ def parse():
     p.add_argument("--ring-power", type=int, default=16)
     p.add_argument("--batch", type=int, default=256)
     p.add_argument("--forms", default="f64,f32split,f64_dropin")
-    p.add_argument("--a-limbs", default="4x6", help="a limb split, bits x count")
+    p.add_argument("--a-limbs", default="4x5", help="a limb split, bits x count")
+    p.add_argument("--a-center", type=int, default=19,
+                   help="FP32 split of a - 2^c plus the exact 2^c (J b) limb (0: split a as is)")
     p.add_argument("--cpu-seconds", type=float, default=10.0)
     p.add_argument("--rb-f64", type=int, default=0, help="radix bits of the FP64 kernel (0: tune)")
     p.add_argument("--rb-f32", type=int, default=0, help="radix bits of the FP32 kernel (0: tune)")
@@ -270,6 +273,7 @@ def main():
             wa, ka = (int(x) for x in args.a_limbs.split("x"))
             plan = G.PolymulPlan(rp, G.PREC_F32_SPLIT, max_batch=batch, a_limb_bits=wa, a_limbs=ka,
                                  stream=sid)
+            plan.a_center_log2 = args.a_center
             out = torch.empty(batch * N, dtype=torch.int64, device=dev)
             err = torch.empty(2 * batch, dtype=torch.float64, device=dev)
             tune = {}
@@ -289,9 +293,12 @@ def main():
             e = err.view(batch, 2)
             res["f32split"] = {"ms": ms, "ms_transform": ms_t, "ms_combine": ms_c,
                                "radix_bits": plan.radix_bits, "tune_ms": tune,
-                               "limbs": "a:%dx%d-bit" % (ka, wa),
+                               "limbs": "a%s:%dx%d-bit%s" % (
+                                   " - 2^%d" % args.a_center if args.a_center else "", ka, wa,
+                                   " + 2^%d (J b) limb" % args.a_center if args.a_center else ""),
                                "err_max": float(e[:, 0].max()),
-                               "err_rms": float(torch.sqrt(e[:, 1].sum() / (batch * N * ka)))}
+                               "err_rms": float(torch.sqrt(e[:, 1].sum() /
+                                                           (batch * N * (ka + (1 if args.a_center else 0)))))}
             outs["f32split"] = out.clone()
             plan.close()
         if "f64_dropin" in forms:

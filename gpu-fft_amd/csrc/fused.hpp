@@ -51,12 +51,16 @@ template <typename T> struct ProdArgs {
     const void* a;
     const void* b;
     long long in_stride;   // input elements per product
-    cplx<T>* scratch;      // [batch][ka*kb][L] partial results (SB > 0)
+    cplx<T>* scratch;      // [batch][limb products][L] partial results (SB > 0)
     T* out;                // direct output (SB == 0, PL_FOLD)
     long long out_stride;
     cplx<T> scale;
     int wa, ka, wb, kb;    // limb split of a and b (PL_FOLD_LIMB)
     long long oa, ob;      // digit offsets: sum_{k < K-1} 2^(w-1) 2^(w k)
+    int ca;                // PL_FOLD_LIMB: a is split as a - 2^ca (0: as is); the
+                           // product then gains the limb 2^ca J b (J = all ones),
+                           // run after the ka digit limbs
+    const cplx<T>* jspec;  // forward FFNT spectrum of J (L entries, bit-reversed order)
     int* arrive;           // per-product arrival counters (in-kernel combine)
 };
 
@@ -64,10 +68,10 @@ template <typename T> struct ProdArgs {
 // the rest): with o = sum_{j < k-1} 2^(w-1) 2^(w j), the unsigned base-2^w
 // digits u_j of v + o give d_j = u_j - 2^(w-1) for j < k-1, and the top digit
 // is (v + o) >> (w (k-1)) (floor).
-__device__ __forceinline__ int limb_digit(int v, int w, int i, int k, long long o)
+__device__ __forceinline__ int limb_digit(long long v, int w, int i, int k, long long o)
 {
-    if (k == 1) return v;
-    const long long u = (long long)v + o;
+    if (k == 1) return (int)v;
+    const long long u = v + o;
     if (i == k - 1) return (int)(u >> (w * (k - 1)));
     return (int)((u >> (w * i)) & ((1LL << w) - 1)) - (1 << (w - 1));
 }
@@ -87,7 +91,7 @@ template <typename T, int PL, int E, int HB> constexpr int in_vec()
 // every workgroup of its product, each limb's digits by every limb pass).
 template <typename T, int PL, int V>
 __device__ __forceinline__ void prod_input_vec(rsrc_t x, int L, int w, int limb, int k, long long o,
-                                               int P, T* re, T* im)
+                                               int center, int P, T* re, T* im)
 {
     if constexpr (PL == PL_FOLD) {
         static_assert(sizeof(T) == 8 && V == 2, "the FP64 form reads doubles in pairs");
@@ -100,16 +104,16 @@ __device__ __forceinline__ void prod_input_vec(rsrc_t x, int L, int w, int limb,
         const u32x4 i = __builtin_amdgcn_raw_buffer_load_b128(x, (unsigned)(P + L) * 4u, 0, 0);
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            re[v] = (T)limb_digit((int)r[v], w, limb, k, o);
-            im[v] = (T)limb_digit((int)i[v], w, limb, k, o);
+            re[v] = (T)limb_digit((long long)(int)r[v] - center, w, limb, k, o);
+            im[v] = (T)limb_digit((long long)(int)i[v] - center, w, limb, k, o);
         }
     } else {
         const u32x2 r = __builtin_amdgcn_raw_buffer_load_b64(x, (unsigned)P * 4u, 0, 0);
         const u32x2 i = __builtin_amdgcn_raw_buffer_load_b64(x, (unsigned)(P + L) * 4u, 0, 0);
 #pragma unroll
         for (int v = 0; v < 2; v++) {
-            re[v] = (T)limb_digit((int)r[v], w, limb, k, o);
-            im[v] = (T)limb_digit((int)i[v], w, limb, k, o);
+            re[v] = (T)limb_digit((long long)(int)r[v] - center, w, limb, k, o);
+            im[v] = (T)limb_digit((long long)(int)i[v] - center, w, limb, k, o);
         }
     }
 }
@@ -238,10 +242,12 @@ __device__ __forceinline__ void top_gs(cplx<T>* y, const ProdArgs<T>& a, int q)
 
 // Gathers the sub-problem's input into LDS: fold + twist + row q of the top
 // network for every position c < 2^S, then the HB-bit top step (top_ct).
+// ones: the operand is the all-ones polynomial J (no loads); center: the
+// limb form splits x - center.
 template <typename T, int S, int SB, int E, int PL, int HB>
 __device__ __forceinline__ void load_sub(cplx<T>* lds, const ProdArgs<T>& a, rsrc_t x,
                                          const cplx<T>* tws, int w, int limb, int k, long long o,
-                                         const cplx<T>* C, int q, int tid)
+                                         int center, bool ones, const cplx<T>* C, int q, int tid)
 {
     constexpr int NT = (1 << S) >> E;
     constexpr int Q = 1 << SB;
@@ -258,9 +264,17 @@ __device__ __forceinline__ void load_sub(cplx<T>* lds, const ProdArgs<T>& a, rsr
         for (int m = 0; m < NP; m++) {
             const int c0 = (tid + (g0 + m * NGH) * NT) * V;
             T re[Q][V], im[Q][V];
+            if (ones) {
 #pragma unroll
-            for (int j = 0; j < Q; j++)
-                prod_input_vec<T, PL, V>(x, L, w, limb, k, o, (j << S) | c0, re[j], im[j]);
+                for (int j = 0; j < Q; j++)
+#pragma unroll
+                    for (int v = 0; v < V; v++) re[j][v] = im[j][v] = T(1);
+            } else {
+#pragma unroll
+                for (int j = 0; j < Q; j++)
+                    prod_input_vec<T, PL, V>(x, L, w, limb, k, o, center, (j << S) | c0, re[j],
+                                             im[j]);
+            }
 #pragma unroll
             for (int v = 0; v < V; v++) {
                 T ax = T(0), ay = T(0);
@@ -385,7 +399,8 @@ __global__ __launch_bounds__((1 << S) >> E) void product_sub_kernel(ProdArgs<T> 
     TileGeo gi = g;
     gi.tab = make_rsrc(a.itab);
     constexpr int NST = (S + RB - 1) / RB;
-    const int nprod = a.ka * a.kb;
+    const int nla = a.ka + (a.ca ? 1 : 0);  // a limbs: ka digits (+ J)
+    const int nprod = nla * a.kb;
     const long long L = 1LL << a.n;
 
     const rsrc_t xa = make_rsrc(reinterpret_cast<const char*>(a.a) +
@@ -394,7 +409,7 @@ __global__ __launch_bounds__((1 << S) >> E) void product_sub_kernel(ProdArgs<T> 
                                 p * a.in_stride * (PL == PL_FOLD ? (long long)sizeof(T) : 4LL));
     fill_twist_split<T, S>(tws, a.twist, tid, NT);  // visible after the first barrier below
     // compile-time single limb for the FP64 form
-    const int KA = PL == PL_FOLD ? 1 : a.ka;
+    const int KA = PL == PL_FOLD ? 1 : nla;
     const int KB = PL == PL_FOLD ? 1 : a.kb;
 
     cplx<T> C[Q];
@@ -412,24 +427,36 @@ __global__ __launch_bounds__((1 << S) >> E) void product_sub_kernel(ProdArgs<T> 
         int t = tid;
         asm volatile("" : "+v"(t));
         __syncthreads();
-        load_sub<T, S, SB, E, PL, HB>(lds, a, xb, tws, a.wb, jb, KB, a.ob, C, q, t);
+        load_sub<T, S, SB, E, PL, HB>(lds, a, xb, tws, a.wb, jb, KB, a.ob, 0, false, C, q, t);
         run_steps_ct<T, MODE_STD_FWD, E, NT, S, RB, FS0, NST - 1, true>(lds, pf, g, t, a.wc);
         last_forward_step<T, E, RB, NT>(lds, pf, g, t, breg, a.wc);
         for (int ia = 0; ia < KA; ia++) {
             int u = tid;
             asm volatile("" : "+v"(u));
             __syncthreads();
-            load_sub<T, S, SB, E, PL, HB>(lds, a, xa, tws, a.wa, ia, KA, a.oa, C, q, u);
-            run_steps_ct<T, MODE_STD_FWD, E, NT, S, RB, FS0, NST - 1, true>(lds, pf, g, u, a.wc);
+            // the J limb (ia == ka) needs no transform: its spectrum is a.jspec
+            const bool jl = PL != PL_FOLD && ia >= a.ka;
+            if (!jl) {
+                load_sub<T, S, SB, E, PL, HB>(lds, a, xa, tws, a.wa, ia, a.ka, a.oa,
+                                              a.ca ? (1 << a.ca) : 0, false, C, q, u);
+                run_steps_ct<T, MODE_STD_FWD, E, NT, S, RB, FS0, NST - 1, true>(lds, pf, g, u,
+                                                                               a.wc);
+            }
             // last forward step, product, first inverse step: in registers,
             // one radix-2^RB group at a time
 #pragma unroll
             for (int k = 0; k < (1 << (E - RB)); k++) {
                 const int rbase = (u + k * NT) << RB;
                 cplx<T> v[1 << RB];
+                if (jl) {
+                    const rsrc_t js = make_rsrc(a.jspec);
 #pragma unroll
-                for (int j = 0; j < (1 << RB); j++) v[j] = lds[swz(rbase | j)];
-                group_stages_derived<T, MODE_STD_FWD, RB>(v, pf, g, rbase, 0, 0, a.wc);
+                    for (int j = 0; j < (1 << RB); j++) v[j] = bldc<T>(js, g.hi_pos | rbase | j);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < (1 << RB); j++) v[j] = lds[swz(rbase | j)];
+                    group_stages_derived<T, MODE_STD_FWD, RB>(v, pf, g, rbase, 0, 0, a.wc);
+                }
 #pragma unroll
                 for (int j = 0; j < (1 << RB); j++) v[j] = v[j] * breg[(k << RB) + j];
                 group_stages_derived<T, MODE_STD_INV, RB>(v, pinv, gi, rbase, 0, 0, a.wci);
@@ -572,7 +599,7 @@ __global__ __launch_bounds__(256) void f64_combine_kernel(F64CombineArgs a)
 }
 
 template <int SB> struct CombineArgs {
-    int n, S, batch, nprod, kb, wa, wb;
+    int n, S, batch, nprod, ka, kb, wa, wb, ca;
     const cplx<float>* scratch;
     const cplx<float>* itab;
     const cplx<float>* untwist;
@@ -613,7 +640,8 @@ __global__ __launch_bounds__(256) void limb_combine_kernel(CombineArgs<SB> a)
                 y[j1] = (U - V) * w;
             }
         }
-        const int shift = a.wa * (l / a.kb) + a.wb * (l % a.kb);
+        const int ia = l / a.kb, jb = l % a.kb;  // ia == ka: the J limb, weight 2^ca
+        const int shift = (ia < a.ka ? a.wa * ia : a.ca) + a.wb * jb;
 #pragma unroll
         for (int j = 0; j < Q; j++) {
             const int P = (j << a.S) | c;

@@ -26,6 +26,8 @@ struct gpufft_polymul_plan_s {
     int precision;
     int max_batch;
     int wa, ka, wb, kb;
+    int ca;          // FP32 split: a split as a - 2^ca (0: none), plus the limb 2^ca J b
+    void* jspec;     // FP32 split with ca: forward FFNT spectrum of J (complex float, L)
     int S, SB;       // sub-problem split
     int rb;          // radix bits per LDS step of the product kernel
     int cmb;         // FP64, SB > 0: 1 = top stages in the product kernel, 0 = separate pass (default)
@@ -58,6 +60,12 @@ __global__ void zero_kernel(double* p, long long n)
 {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i < n) p[i] = 0.0;
+}
+
+__global__ void fill_ones(float* p, long long n)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 1.0f;
 }
 
 __global__ void zero_ints(int* p, int n)
@@ -143,8 +151,8 @@ template <typename T> int make_tables(gpufft_polymul_plan_s* pl, hipStream_t st)
 void release(gpufft_polymul_plan_s* pl)
 {
     if (!pl) return;
-    void* ptrs[8] = {pl->root, pl->iroot, pl->twist, pl->untwist, pl->wc, pl->wci, pl->scratch,
-                     pl->arrive};
+    void* ptrs[9] = {pl->root, pl->iroot, pl->twist, pl->untwist, pl->wc, pl->wci, pl->scratch,
+                     pl->arrive, pl->jspec};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete pl;
@@ -174,6 +182,8 @@ int launch_products(const gpufft_polymul_plan_s* pl, const void* a, const void* 
     pa.scale = mk<T>((T)(1.0 / (double)(1LL << pl->n)), (T)0);
     pa.wa = pl->wa;
     pa.ka = pl->ka;
+    pa.ca = pl->ca;
+    pa.jspec = (const cplx<T>*)pl->jspec;
     pa.wb = pl->wb;
     pa.kb = pl->kb;
     auto offset = [](int w, int k) {
@@ -257,7 +267,9 @@ int gpufft_polymul_plan_create(gpufft_polymul_plan* out, int ring_power, int pre
     const bool need_scratch = precision == GPUFFT_PREC_F32_SPLIT || pl->SB > 0;
     if (need_scratch) {
         size_t elem = precision == GPUFFT_PREC_F64 ? 16 : 8;
-        pl->scratch_bytes = (size_t)max_batch * pl->ka * pl->kb * (elem << n);
+        // FP32 split: room for the optional J limb (gpufft_polymul_plan_set_a_center)
+        const size_t limbs = precision == GPUFFT_PREC_F32_SPLIT ? (size_t)(pl->ka + 1) * pl->kb : 1;
+        pl->scratch_bytes = (size_t)max_batch * limbs * (elem << n);
         hipError_t e = hipMalloc(&pl->scratch, pl->scratch_bytes);
         if (e != hipSuccess) {
             release(pl);
@@ -373,7 +385,9 @@ int gpufft_polymul_i32_phases(gpufft_polymul_plan plan, const int32_t* a, const 
         ca.n = plan->n;                                                                    \
         ca.S = plan->S;                                                                    \
         ca.batch = nb;                                                                     \
-        ca.nprod = plan->ka * plan->kb;                                                    \
+        ca.nprod = (plan->ka + (plan->ca ? 1 : 0)) * plan->kb;                             \
+        ca.ka = plan->ka;                                                                  \
+        ca.ca = plan->ca;                                                                  \
         ca.kb = plan->kb;                                                                  \
         ca.wa = plan->wa;                                                                  \
         ca.wb = plan->wb;                                                                  \
@@ -435,6 +449,47 @@ int gpufft_polymul_plan_set_combine(gpufft_polymul_plan plan, int in_kernel)
 int gpufft_polymul_plan_get_combine(gpufft_polymul_plan plan)
 {
     return plan ? plan->cmb : -1;
+}
+
+int gpufft_polymul_plan_set_a_center(gpufft_polymul_plan plan, int center_log2)
+{
+    const char* who = "gpufft_polymul_plan_set_a_center";
+    if (!plan) return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": plan is NULL");
+    if (plan->precision != GPUFFT_PREC_F32_SPLIT)
+        return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": only GPUFFT_PREC_F32_SPLIT plans");
+    if (center_log2 < 0 || center_log2 > 30)
+        return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": center_log2 must be 0..30");
+    if (center_log2 > 0 && !plan->jspec) {
+        // spectrum of J (all ones), once per plan: the GPU_FFNT forward of the
+        // plan's own tables, so it is the transform the product kernel applies
+        const long long N = 2LL << plan->n, L = 1LL << plan->n;
+        hipError_t e = hipMalloc(&plan->jspec, sizeof(cplx<float>) * (size_t)L);
+        if (e != hipSuccess) return fail_hip(e, "gpufft_polymul_plan_set_a_center: hipMalloc");
+        float* ones = nullptr;
+        e = hipMalloc((void**)&ones, sizeof(float) * (size_t)N);
+        if (e != hipSuccess) {
+            (void)hipFree(plan->jspec);
+            plan->jspec = nullptr;
+            return fail_hip(e, "gpufft_polymul_plan_set_a_center: hipMalloc");
+        }
+        hipLaunchKernelGGL(fill_ones, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, 0, ones, N);
+        gpufft_config cfg{};
+        cfg.n_power = plan->ring_power;
+        cfg.fft_type = GPUFFT_FORWARD;
+        cfg.stream = nullptr;
+        int st = gpufft_ffnt_f32(ones, plan->jspec, plan->twist, plan->root, &cfg, 1, 0);
+        hipError_t e2 = hipDeviceSynchronize();
+        (void)hipFree(ones);
+        if (st) return st;
+        if (e2 != hipSuccess) return fail_hip(e2, "gpufft_polymul_plan_set_a_center: J spectrum");
+    }
+    plan->ca = center_log2;
+    return ok();
+}
+
+int gpufft_polymul_plan_get_a_center(gpufft_polymul_plan plan)
+{
+    return plan ? plan->ca : -1;
 }
 
 }  // extern "C"

@@ -90,6 +90,8 @@ _SIGS = {
     "gpufft_polymul_plan_get_radix_bits": [_P],
     "gpufft_polymul_plan_set_combine": [_P, _I],
     "gpufft_polymul_plan_get_combine": [_P],
+    "gpufft_polymul_plan_set_a_center": [_P, _I],
+    "gpufft_polymul_plan_get_a_center": [_P],
     "gpufft_polymul_i32_phases": [_P, _P, _P, _P, _I, _P, _I, _P],
     "gpufft_ckks_encode_f64": [_P, _P, _P, _I, ctypes.c_double, _I, _P, _P],
     "gpufft_ckks_decode_f64": [_P, _P, _P, _I, ctypes.c_double, _I, _P],
@@ -267,6 +269,16 @@ class PolymulPlan:
     @combine_in_kernel.setter
     def combine_in_kernel(self, v):
         check(_lib.gpufft_polymul_plan_set_combine(self.handle, 1 if v else 0))
+
+    @property
+    def a_center_log2(self):
+        """FP32-split plans: a is split as a - 2^c (0: as is), plus the exact
+        correction limb 2^c (J b)."""
+        return _lib.gpufft_polymul_plan_get_a_center(self.handle)
+
+    @a_center_log2.setter
+    def a_center_log2(self, c):
+        check(_lib.gpufft_polymul_plan_set_a_center(self.handle, c))
 
     @property
     def scratch_bytes(self):

@@ -183,6 +183,14 @@ int gpufft_polymul_plan_get_radix_bits(gpufft_polymul_plan plan);
  * Other plans ignore the setting (get returns 0).  Same results either way. */
 int gpufft_polymul_plan_set_combine(gpufft_polymul_plan plan, int in_kernel);
 int gpufft_polymul_plan_get_combine(gpufft_polymul_plan plan);
+/* GPUFFT_PREC_F32_SPLIT plans: split a - 2^center_log2 instead of a (0, the
+ * default: a as is), and add the exact correction 2^center_log2 * (J * b),
+ * J the all-ones polynomial, as one more limb product (no operand loads).
+ * For a in [0, 2^20) with 4-bit limbs, center 19 makes 5 limbs enough (every
+ * digit in [-8, 8]) where a as is needs 6: one limb product less per call.
+ * Results are the same exact products; only the split changes. */
+int gpufft_polymul_plan_set_a_center(gpufft_polymul_plan plan, int center_log2);
+int gpufft_polymul_plan_get_a_center(gpufft_polymul_plan plan);
 int gpufft_polymul_f64(gpufft_polymul_plan plan, const double* a, const double* b, double* out,
                        int batch, void* stream);
 int gpufft_polymul_i32(gpufft_polymul_plan plan, const int32_t* a, const int32_t* b, int64_t* out,

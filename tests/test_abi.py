@@ -75,6 +75,8 @@ def test_polymul_plan_validation_on_host():
         G.PolymulPlan(12, G.PREC_F64, 0, stream=0)
     assert G.lib().gpufft_polymul_plan_set_combine(None, 1) == G.ERR_NULL_POINTER
     assert G.lib().gpufft_polymul_plan_get_combine(None) == -1
+    assert G.lib().gpufft_polymul_plan_set_a_center(None, 19) == G.ERR_NULL_POINTER
+    assert G.lib().gpufft_polymul_plan_get_a_center(None) == -1
 
 
 def test_table_generation_validation_on_host():

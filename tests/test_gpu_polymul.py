@@ -122,10 +122,12 @@ def test_polymul_f64_combine_forms_agree():
 
 # ---- FP32-split fused product ----
 SPLITS = {
-    # (a_limb_bits, a_limbs, b_limb_bits, b_limbs)
+    # (a_limb_bits, a_limbs, b_limb_bits, b_limbs[, a center log2])
     "a5x5": (5, 5, 0, 1),
     "a4x6": (4, 6, 0, 1),
     "a7x3_b4x2": (7, 3, 4, 2),
+    "a4x5c19": (4, 5, 0, 1, 19),
+    "a7x3c20_b4x2": (7, 3, 4, 2, 20),
 }
 
 
@@ -138,9 +140,12 @@ def test_polymul_f32_split_exact(ring_power, split, rb):
     N = 1 << ring_power
     batch = 5
     a, b = _operands(N, batch, SEED_BASE + 33 + ring_power)
-    wa, ka, wb, kb = SPLITS[split]
+    wa, ka, wb, kb = SPLITS[split][:4]
     plan = G.PolymulPlan(ring_power, G.PREC_F32_SPLIT, batch, wa, ka, wb, kb)
     plan.radix_bits = rb
+    if len(SPLITS[split]) > 4:
+        plan.a_center_log2 = SPLITS[split][4]
+        assert plan.a_center_log2 == SPLITS[split][4]
     out = torch.empty(batch * N, dtype=torch.int64, device="cuda")
     err = torch.empty(batch * 2, dtype=torch.float64, device="cuda")
     plan.multiply_i32(to_dev(a.astype(np.int32).reshape(-1)), to_dev(b.astype(np.int32).reshape(-1)),
