@@ -82,4 +82,23 @@ template <typename T> __device__ __forceinline__ void stc(cplx<T>* p, cplx<T> v)
     *p = v;
 }
 
+// Streaming load (non-temporal hint): data read once.
+template <typename T> __device__ __forceinline__ cplx<T> ldc_nt(const cplx<T>* p)
+{
+    typedef T vec2 __attribute__((ext_vector_type(2)));
+    const vec2 w = __builtin_nontemporal_load(reinterpret_cast<const vec2*>(p));
+    return mk<T>(w.x, w.y);
+}
+
+// Streaming store (non-temporal hint): data written once and read back by a
+// later launch, kept from displacing the lines the current launch re-reads.
+template <typename T> __device__ __forceinline__ void stc_nt(cplx<T>* p, cplx<T> v)
+{
+    typedef T vec2 __attribute__((ext_vector_type(2)));
+    vec2 w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<vec2*>(p));
+}
+
 }  // namespace gfa

@@ -497,7 +497,7 @@ __global__ __launch_bounds__((1 << S) >> E) void product_sub_kernel(ProdArgs<T> 
                     for (int m = 0; m < NP; m++) z[m] = lds[swz(c + (m << (S - HB)))];
                     if constexpr (HB > 0) top_gs<T, S, HB>(z, a, q);
 #pragma unroll
-                    for (int m = 0; m < NP; m++) stc(dst + c + (m << (S - HB)), z[m]);
+                    for (int m = 0; m < NP; m++) stc_nt(dst + c + (m << (S - HB)), z[m]);
                 }
             }
         }
@@ -567,8 +567,8 @@ __global__ __launch_bounds__(256) void f64_combine_kernel(F64CombineArgs a)
     cplx<double> y[2][Q];
 #pragma unroll
     for (int j = 0; j < Q; j++) {
-        y[0][j] = ldc(src + ((long long)j << a.S) + c);
-        y[1][j] = ldc(src + ((long long)j << a.S) + c + 1);
+        y[0][j] = ldc_nt(src + ((long long)j << a.S) + c);
+        y[1][j] = ldc_nt(src + ((long long)j << a.S) + c + 1);
     }
     double* o = a.out + p * a.out_stride;
 #pragma unroll
@@ -593,8 +593,8 @@ __global__ __launch_bounds__(256) void f64_combine_kernel(F64CombineArgs a)
         for (int v = 0; v < 2; v++)
             z[v] = mk<double>(y[v][j].x * a.scale, y[v][j].y * a.scale) *
                    (ut[128 + ((P + v) >> 7)] * ut[(P + v) & 127]);
-        stc(reinterpret_cast<cplx<double>*>(o + P), mk<double>(z[0].x, z[1].x));
-        stc(reinterpret_cast<cplx<double>*>(o + P + L), mk<double>(z[0].y, z[1].y));
+        stc_nt(reinterpret_cast<cplx<double>*>(o + P), mk<double>(z[0].x, z[1].x));
+        stc_nt(reinterpret_cast<cplx<double>*>(o + P + L), mk<double>(z[0].y, z[1].y));
     }
 }
 
@@ -627,7 +627,7 @@ __global__ __launch_bounds__(256) void limb_combine_kernel(CombineArgs<SB> a)
         const cplx<float>* src = a.scratch + (p * a.nprod + l) * L;
         cplx<float> y[Q];
 #pragma unroll
-        for (int j = 0; j < Q; j++) y[j] = ldc(src + ((long long)j << a.S) + c);
+        for (int j = 0; j < Q; j++) y[j] = ldc_nt(src + ((long long)j << a.S) + c);
 #pragma unroll
         for (int i = 0; i < SB; i++) {  // Gentleman-Sande, bits S .. n-1
 #pragma unroll
@@ -658,8 +658,8 @@ __global__ __launch_bounds__(256) void limb_combine_kernel(CombineArgs<SB> a)
 #pragma unroll
     for (int j = 0; j < Q; j++) {
         const int P = (j << a.S) | c;
-        o[P] = acc[j][0];
-        o[P + L] = acc[j][1];
+        __builtin_nontemporal_store(acc[j][0], o + P);
+        __builtin_nontemporal_store(acc[j][1], o + P + L);
     }
     if (a.err) {
         // one wavefront holds 64 consecutive positions of one product
