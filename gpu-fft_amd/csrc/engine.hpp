@@ -58,6 +58,7 @@ template <typename T> struct PassArgs {
     int batch;   // transforms in the launch
     int nega;    // X^N+1 twiddle base (standard modes only)
     int do_scale;
+    int nt_in, nt_out;   // streaming (non-temporal) source loads / destination stores
     cplx<T> scale;
     const cplx<T>* tab;  // root-of-unity table (reference layout)
     // input
@@ -75,23 +76,52 @@ template <typename T> struct PassArgs {
 // XOR of the next two 4-bit fields of the index.
 __device__ __forceinline__ int swz(int i) { return i ^ (((i >> 4) ^ (i >> 8)) & 15); }
 
+// Source and destination accesses of a pass: streaming (non-temporal) when
+// the buffer is read or written once by this transform (first-pass source,
+// last-pass destination), so that it does not displace cached tables and
+// intermediate data; cached otherwise.
+template <typename U> __device__ __forceinline__ U ld_s(const U* p, bool nt)
+{
+    return nt ? __builtin_nontemporal_load(p) : *p;
+}
+template <typename U> __device__ __forceinline__ void st_s(U* p, U v, bool nt)
+{
+    if (nt)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
+template <typename T> __device__ __forceinline__ cplx<T> ldc_s(const cplx<T>* p, bool nt)
+{
+    return nt ? ldc_nt(p) : ldc(p);
+}
+template <typename T> __device__ __forceinline__ void stc_s(cplx<T>* p, cplx<T> v, bool nt)
+{
+    if (nt)
+        stc_nt(p, v);
+    else
+        stc(p, v);
+}
+
 template <typename T, int LK>
 __device__ __forceinline__ cplx<T> load_elem(const PassArgs<T>& a, long long b, int p)
 {
+    const bool nt = a.nt_in;
     if constexpr (LK == LD_CPLX) {
-        return ldc(reinterpret_cast<const cplx<T>*>(a.src) + b * a.src_stride + p);
+        return ldc_s(reinterpret_cast<const cplx<T>*>(a.src) + b * a.src_stride + p, nt);
     } else if constexpr (LK == LD_REAL) {
-        return mk<T>(reinterpret_cast<const T*>(a.src)[b * a.src_stride + p], T(0));
+        return mk<T>(ld_s(reinterpret_cast<const T*>(a.src) + b * a.src_stride + p, nt), T(0));
     } else if constexpr (LK == LD_MUL) {
-        cplx<T> u = ldc(reinterpret_cast<const cplx<T>*>(a.src) + b * a.src_stride + p);
-        cplx<T> v = ldc(reinterpret_cast<const cplx<T>*>(a.src2) + b * a.src_stride + p);
+        cplx<T> u = ldc_s(reinterpret_cast<const cplx<T>*>(a.src) + b * a.src_stride + p, nt);
+        cplx<T> v = ldc_s(reinterpret_cast<const cplx<T>*>(a.src2) + b * a.src_stride + p, nt);
         return u * v;
     } else if constexpr (LK == LD_FOLD) {  // z = (f[p] + i f[p+L]) * twist[p]
         const T* f = reinterpret_cast<const T*>(a.src) + b * a.src_stride;
         int L = 1 << a.n;
-        return mk<T>(f[p], f[p + L]) * ldc(a.ltab + p);
+        return mk<T>(ld_s(f + p, nt), ld_s(f + p + L, nt)) * ldc(a.ltab + p);
     } else {  // LD_SPLIT_HI / LD_SPLIT_LO: x = hi + lo, hi = float(x), lo = float(x - hi)
-        const cplx<double> x = ldc(reinterpret_cast<const cplx<double>*>(a.src) + b * a.src_stride + p);
+        const cplx<double> x =
+            ldc_s(reinterpret_cast<const cplx<double>*>(a.src) + b * a.src_stride + p, nt);
         const T hx = (T)x.x, hy = (T)x.y;
         if constexpr (LK == LD_SPLIT_HI) return mk<T>(hx, hy);
         return mk<T>((T)(x.x - (double)hx), (T)(x.y - (double)hy));
@@ -101,21 +131,22 @@ __device__ __forceinline__ cplx<T> load_elem(const PassArgs<T>& a, long long b, 
 template <typename T, int SK>
 __device__ __forceinline__ void store_elem(const PassArgs<T>& a, long long b, int p, cplx<T> z)
 {
+    const bool nt = a.nt_out;
     if (a.do_scale) z = z * a.scale;
     if constexpr (SK == ST_CPLX) {
-        stc(reinterpret_cast<cplx<T>*>(a.dst) + b * a.dst_stride + p, z);
+        stc_s(reinterpret_cast<cplx<T>*>(a.dst) + b * a.dst_stride + p, z, nt);
     } else if constexpr (SK == ST_REAL) {
-        reinterpret_cast<T*>(a.dst)[b * a.dst_stride + p] = z.x;
+        st_s(reinterpret_cast<T*>(a.dst) + b * a.dst_stride + p, z.x, nt);
     } else if constexpr (SK == ST_UNFOLD) {  // (z * untwist[p]) -> out[p] = re, out[p+L] = im
         z = z * ldc(a.stab + p);
         T* o = reinterpret_cast<T*>(a.dst) + b * a.dst_stride;
         int L = 1 << a.n;
-        o[p] = z.x;
-        o[p + L] = z.y;
+        st_s(o + p, z.x, nt);
+        st_s(o + p + L, z.y, nt);
     } else {  // ST_SUM_F64
-        const cplx<T> h = ldc(a.stab + b * a.dst_stride + p);
-        stc(reinterpret_cast<cplx<double>*>(a.dst) + b * a.dst_stride + p,
-            mk<double>((double)h.x + (double)z.x, (double)h.y + (double)z.y));
+        const cplx<T> h = ldc_s(a.stab + b * a.dst_stride + p, nt);
+        stc_s(reinterpret_cast<cplx<double>*>(a.dst) + b * a.dst_stride + p,
+              mk<double>((double)h.x + (double)z.x, (double)h.y + (double)z.y), nt);
     }
 }
 

@@ -135,6 +135,8 @@ int run_transform(int mode, int n, int batch, int nega, const cplx<T>* tab,
         a.nega = nega;
         a.tab = tab;
         int lk = LD_CPLX, sk = ST_CPLX;
+        a.nt_in = first ? 1 : 0;
+        a.nt_out = last ? 1 : 0;
         if (first) {
             lk = io.lk;
             a.src = io.src;
