@@ -49,6 +49,10 @@ def parse():
                         "(one JSON line per point) instead of the headline workload")
     p.add_argument("--surface-logn", default="12-23", help="ring size logN range of --surface")
     p.add_argument("--surface-dtype", default="f64", choices=["f64", "f32"])
+    p.add_argument("--batched-uneven", type=int, default=0, metavar="COUNT",
+                   help="time COUNT products of uneven ring sizes (2^10..2^17, half linear, half "
+                        "negacyclic) in one persistent launch, FP32 split, against FP64 launches "
+                        "per ring size; one JSON line")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only run 3 device copies of 512 MiB (reference kernel for HBM counters)")
     return p.parse_args()
@@ -205,10 +209,102 @@ def surface(args):
             torch.cuda.synchronize()
 
 
+def batched_uneven(args):
+    """Batched uneven form: `count` products, ring sizes 2^e with e uniform in
+    {10..17}, even index linear (C_C, zero-padded cyclic), odd negacyclic,
+    coefficients uniform in [-2^8, 2^8), seeded.  One persistent launch
+    replayed as a HIP graph (FP32 split, a and b in 2 limbs of 5 bits, radix
+    and grid tuned), against one launch per (kind, ring size) group in FP64
+    (fused negacyclic plans; drop-in GPU_FFT C_C pairs for the linear
+    products).  Rounded products of the persistent launch are checked
+    against the FP64 launches."""
+    import numpy as np
+    import torch
+
+    import gpufft_amd as G
+    from gpufft_amd import autotune as A
+    from gpufft_amd import synthetic as S
+    count = args.batched_uneven
+    e = S.uniform(S.SEED_BASE + 4, 10, 18, count)
+    kinds = [G.PRODUCT_LINEAR if i % 2 == 0 else G.PRODUCT_NEGACYCLIC for i in range(count)]
+    descs, n_in, n_out = G.pack_products(kinds, e)
+    a = S.uniform(S.SEED_BASE + 4 + 0x100, -256, 256, n_in).astype(np.int32)
+    b = S.uniform(S.SEED_BASE + 4 + 0x200, -256, 256, n_in).astype(np.int32)
+    dev = torch.device("cuda:0")
+    da, db = torch.from_numpy(a).to(dev), torch.from_numpy(b).to(dev)
+    out = torch.zeros(n_out, dtype=torch.int64, device=dev)
+    err = torch.zeros(count, dtype=torch.float64, device=dev)
+    s = torch.cuda.Stream(device=dev)
+    sid = s.cuda_stream
+    plan = G.BatchedPlan(descs, 5, 2, 5, 2, stream=sid)
+    with torch.cuda.stream(s):
+        cand, _ = A.tune_batched(plan, lambda: plan.execute(da, db, out, stream=sid), s,
+                                 key=A.signature("batched_uneven", count), use_db=False)
+        plan.execute(da, db, out, err, stream=sid)
+        ms = A.time_ms(lambda: plan.graph_launch(da, db, out, stream=sid), s, reps=max(3, args.steps // 4))
+    s.synchronize()
+    got = out.cpu().numpy()
+    alg = sum(2 * (1 << int(d.ring_power)) * 4 + G.output_length(d.kind, d.ring_power) * 8
+              for d in descs)
+    # FP64: one launch (pair) per (kind, ring size) group, same products
+    groups = {}
+    for i, d in enumerate(descs):
+        groups.setdefault((d.kind, int(d.ring_power)), []).append(i)
+    t64 = 0.0
+    exact = True
+    with torch.cuda.stream(s):
+        for (k, r), idx in sorted(groups.items()):
+            N, B = 1 << r, len(idx)
+            xa = np.concatenate([a[descs[i].a_offset:descs[i].a_offset + N] for i in idx])
+            xb = np.concatenate([b[descs[i].b_offset:descs[i].b_offset + N] for i in idx])
+            olen = G.output_length(k, r)
+            if k == G.PRODUCT_NEGACYCLIC:
+                pl = G.PolymulPlan(r, G.PREC_F64, max_batch=B, stream=sid)
+                x = torch.from_numpy(xa.astype(np.float64)).to(dev)
+                y = torch.from_numpy(xb.astype(np.float64)).to(dev)
+                o = torch.empty(B * N, dtype=torch.float64, device=dev)
+                t64 += A.time_ms(lambda: pl.multiply_f64(x, y, o, B, stream=sid), s, reps=3)
+                res = torch.round(o).to(torch.int64).view(B, N)
+                pl.close()
+            else:
+                L = 2 * N
+                tt = G.generate_table(G.TABLE_FFT, r, False, stream=sid)
+                it = G.generate_table(G.TABLE_FFT, r, True, stream=sid)
+                src = torch.zeros(2 * B, L, dtype=torch.complex128, device=dev)
+                src[:B, :N] = torch.from_numpy(xa.astype(np.float64)).to(dev).view(B, N)
+                src[B:, :N] = torch.from_numpy(xb.astype(np.float64)).to(dev).view(B, N)
+                buf = torch.empty_like(src)
+                cf = G.config(r + 1, G.FORWARD, G.X_N_MINUS, stream=sid)
+                ci = G.config(r + 1, G.INVERSE, G.X_N_MINUS, 1.0 / L, stream=sid)
+
+                def run():
+                    buf.copy_(src)
+                    G.GPU_FFT(buf, tt, cf, 2 * B)
+                    G.GPU_FFT(buf, it, ci, B, True)
+                t_copy = A.time_ms(lambda: buf.copy_(src), s, reps=3)
+                t64 += A.time_ms(run, s, reps=3) - t_copy
+                res = torch.round(buf[:B, :olen].real).to(torch.int64)
+            s.synchronize()
+            res = res.cpu().numpy()
+            for j, i in enumerate(idx):
+                o0 = descs[i].out_offset
+                exact = exact and np.array_equal(got[o0:o0 + olen], res[j, :olen])
+    print(json.dumps({"workload": "batched uneven products", "count": count,
+                      "ring_powers": "10..17", "kinds": "half linear (C_C), half negacyclic",
+                      "coefficients": "[-2^8, 2^8)", "fp32_split": "a, b: 2 limbs of 5 bits",
+                      "tuned": list(cand), "persistent_graph_ms": ms,
+                      "products_per_s": count / (ms * 1e-3), "alg_GBps": alg / (ms * 1e-3) / 1e9,
+                      "max_err": float(err.max()), "fp64_per_size_ms": t64,
+                      "size_groups": len(groups), "bit_exact_vs_fp64": bool(exact)}))
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.batched_uneven:
+        batched_uneven(args)
         return
     if args.surface:
         surface(args)

@@ -274,7 +274,7 @@ __device__ void run_item(cplx<float>* lds, const BArgs& ba, const BProd& pr, con
 #pragma unroll 4
                     for (int i = 0; i < EPT; i++) {
                         const int c = u + i * NT;
-                        stc(dst + c, lds[swz(c)]);
+                        stc_nt(dst + c, lds[swz(c)]);
                     }
                 }
             }
