@@ -233,3 +233,57 @@ def test_c2c_negacyclic_product_exact(n_power):
     G.GPU_FFT(d, to_dev(t["inv_rev"]), G.config(n_power, G.INVERSE, G.X_N_PLUS, 1.0 / N), batch, True)
     got = np.round(to_host(d).reshape(2 * batch, N)[:batch].real).astype(np.int64)
     assert np.array_equal(got, O.ntt_negacyclic(a, b))
+
+
+# ---- the top of the drop-in range (n_power 24: 2^24 points, four passes) ----
+@pytest.mark.parametrize("kind", ["c2c", "c2c_nega", "r2r", "ffnt", "special"])
+def test_dropin_max_size(kind):
+    """Every drop-in entry point at n_power = 24 (reference maximum,
+    fft.cu:522-525), batch 1, FP64, against the oracle."""
+    G = _g()
+    import torch
+    n_power = 24
+    rng = np.random.default_rng(7700 + len(kind))
+    L = 1 << n_power
+    if kind in ("c2c", "c2c_nega"):
+        nega = kind == "c2c_nega"
+        tabs = _tables(n_power, nega, "f64")
+        x = rng_complex(rng, L, "f64")
+        d = to_dev(x)
+        rp = G.X_N_PLUS if nega else G.X_N_MINUS
+        G.GPU_FFT(d, to_dev(tabs["rev"]), G.config(n_power, G.FORWARD, rp), 1)
+        ref = O.c2c_forward(x, n_power, 1, tabs["rev"], nega)
+        assert rel_err(to_host(d), ref) < TOL["f64"]
+        G.GPU_FFT(d, to_dev(tabs["inv_rev"]), G.config(n_power, G.INVERSE, rp, 1.0 / L), 1)
+        assert rel_err(to_host(d), x) < 100 * TOL["f64"]
+    elif kind == "r2r":
+        tabs = _tables(n_power, 0, "f64")
+        real = rng.standard_normal(L)
+        dr = to_dev(real)
+        dt = torch.zeros(L, dtype=torch.complex128, device="cuda")
+        G.GPU_FFT_R(dr, dt, to_dev(tabs["rev"]), G.config(n_power, G.FORWARD), 1)
+        ref = O.r2r_forward(real, n_power, 1, tabs["rev"], 0)
+        assert rel_err(to_host(dt), ref) < TOL["f64"]
+        G.GPU_FFT_R(dr, dt, to_dev(tabs["inv_rev"]), G.config(n_power, G.INVERSE, mod_inverse=1.0 / L), 1)
+        assert rel_err(to_host(dr), real) < 100 * TOL["f64"]
+    elif kind == "ffnt":
+        N, Lh = L, L // 2
+        t = O.ffnt_tables(N)
+        real = rng.standard_normal(N)
+        dr = to_dev(real)
+        dt = torch.zeros(Lh, dtype=torch.complex128, device="cuda")
+        G.GPU_FFNT(dr, dt, to_dev(t["twist"]), to_dev(t["rev_root"]), G.config(n_power, G.FORWARD), 1)
+        ref = O.ffnt_forward(real, n_power, 1, t["twist"], t["rev_root"])
+        assert rel_err(to_host(dt), ref) < TOL["f64"]
+        G.GPU_FFNT(dr, dt, to_dev(t["untwist"]), to_dev(t["inv_rev_root"]),
+                   G.config(n_power, G.INVERSE, mod_inverse=1.0 / Lh), 1)
+        assert rel_err(to_host(dr), real) < 100 * TOL["f64"]
+    else:
+        T, Ti = O.special_table(L), O.special_table(L, True)
+        z = rng_complex(rng, L, "f64")
+        d = to_dev(z)
+        G.GPU_Special_FFT(d, to_dev(T), G.config(n_power, G.FORWARD), 1)
+        ref = O.special_forward(z, n_power, 1, T)
+        assert rel_err(to_host(d), ref) < 4 * TOL["f64"]
+        G.GPU_Special_FFT(d, to_dev(Ti), G.config(n_power, G.INVERSE, mod_inverse=1.0 / L), 1)
+        assert rel_err(to_host(d), z) < 100 * TOL["f64"]
