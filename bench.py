@@ -53,6 +53,9 @@ def parse():
                    help="time COUNT products of uneven ring sizes (2^10..2^17, half linear, half "
                         "negacyclic) in one persistent launch, FP32 split, against FP64 launches "
                         "per ring size; one JSON line")
+    p.add_argument("--configs", action="store_true",
+                   help="time the plain-method and special-FFT configurations (C_C latency, "
+                        "R_R batch 64, special FFT batch 512); one JSON line each")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only run 3 device copies of 512 MiB (reference kernel for HBM counters)")
     return p.parse_args()
@@ -298,8 +301,180 @@ def batched_uneven(args):
                       "size_groups": len(groups), "bit_exact_vs_fp64": bool(exact)}))
 
 
+def _eval_mod(coeffs, xs, p=2147483647):
+    """Values mod p of the integer polynomials in the rows of `coeffs` at the
+    points xs (Horner in int64: every intermediate stays below 2^62)."""
+    import numpy as np
+    c = np.mod(coeffs.astype(np.int64), p)
+    out = np.zeros((c.shape[0], len(xs)), np.int64)
+    for k, x in enumerate(xs):
+        acc = np.zeros(c.shape[0], np.int64)
+        for i in range(c.shape[1] - 1, -1, -1):
+            acc = (acc * x + c[:, i]) % p
+        out[:, k] = acc
+    return out
+
+
+def _poly_check(a, b, c, p=2147483647, points=3, seed=1):
+    """c = a * b as polynomials, checked at random points mod p (a false
+    pass has probability below deg/p per point)."""
+    import numpy as np
+    xs = np.random.default_rng(seed).integers(2, p - 1, size=points).tolist()
+    ea, eb, ec = _eval_mod(a, xs, p), _eval_mod(b, xs, p), _eval_mod(c, xs, p)
+    return bool(np.array_equal((ea * eb) % p, ec))
+
+
+def config_suite(args):
+    """Plain-method and special-FFT configurations, one JSON line each:
+      C_C  cyclic product of two polynomials of degree < 2^12, coefficients in
+           [0, 16), zero-padded to 2^13, FP64: GPU time of the two drop-in
+           calls (graph replay), rounded product checked exactly;
+      R_R  batch 64 products of degree < 2^15 inputs zero-padded to 2^16,
+           coefficients in [-2^10, 2^10), FP64: forward-only (128 polynomials)
+           and full multiply, algorithmic GB/s, products checked exactly;
+      special  l = 2^15 slots, batch 512, slots on the unit disk x 2^30: FP64
+           encode (inverse) and decode (forward), and the FP32 hi/lo-split pair,
+           with the split form's slot error against the FP64 results."""
+    import numpy as np
+    import torch
+
+    import gpufft_amd as G
+    from gpufft_amd import synthetic as S
+    dev = torch.device("cuda:0")
+    st = torch.cuda.Stream(device=dev)
+    sid = st.cuda_stream
+
+    def ev_ms(fn, reps):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            fn()
+        e1.record(st)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    def graph_ms(fn, reps=20):
+        fn()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=st):
+            for _ in range(reps):
+                fn()
+        g.replay()
+        torch.cuda.synchronize()
+        return ev_ms(g.replay, 10) / reps
+
+    with torch.cuda.stream(st):
+        # ---- C_C single product, n = 2^12 -> transform 2^13
+        n, L = 1 << 12, 1 << 13
+        a = S.uniform(S.SEED_BASE + 1, 0, 16, n)
+        b = S.uniform(S.SEED_BASE + 1 + 0x100, 0, 16, n)
+        host = np.zeros((2, L), np.complex128)
+        host[0, :n], host[1, :n] = a, b
+        src = torch.from_numpy(host.reshape(-1)).to(dev)
+        buf = torch.empty_like(src)
+        tt = G.generate_table(G.TABLE_FFT, 12, False, stream=sid)
+        it = G.generate_table(G.TABLE_FFT, 12, True, stream=sid)
+        cf = G.config(13, G.FORWARD, G.X_N_MINUS, stream=sid)
+        ci = G.config(13, G.INVERSE, G.X_N_MINUS, 1.0 / L, stream=sid)
+
+        def cc():
+            G.GPU_FFT(buf, tt, cf, 2)
+            G.GPU_FFT(buf, it, ci, 1, True)
+        buf.copy_(src)
+        cc()
+        torch.cuda.synchronize()
+        got = np.round(buf[:L].real.cpu().numpy()).astype(np.int64)
+        exact = np.array_equal(got[:2 * n - 1], np.convolve(a, b)) and not got[2 * n - 1:].any()
+        t_cc = graph_ms(cc)
+        print(json.dumps({"config": "C_C cyclic product, degree < 2^12, [0,16), zero-padded 2^13, f64",
+                          "gpu_us_per_product": t_cc * 1e3, "bit_exact": bool(exact)}), flush=True)
+
+        # ---- R_R batch 64, N = 2^16 (inputs degree < 2^15)
+        B, N, h = 64, 1 << 16, 1 << 15
+        a = S.uniform(S.SEED_BASE + 2, -1024, 1024, B * h).reshape(B, h)
+        b = S.uniform(S.SEED_BASE + 2 + 0x100, -1024, 1024, B * h).reshape(B, h)
+        real = np.zeros((2 * B, N), np.float64)
+        real[:B, :h], real[B:, :h] = a, b
+        rsrc = torch.from_numpy(real.reshape(-1)).to(dev)
+        rbuf = torch.empty_like(rsrc)
+        temp = torch.empty(2 * B * N, dtype=torch.complex128, device=dev)
+        tt = G.generate_table(G.TABLE_FFT, 15, False, stream=sid)
+        it = G.generate_table(G.TABLE_FFT, 15, True, stream=sid)
+        cf = G.config(16, G.FORWARD, G.X_N_MINUS, stream=sid)
+        ci = G.config(16, G.INVERSE, G.X_N_MINUS, 1.0 / N, stream=sid)
+        rbuf.copy_(rsrc)
+        t_fwd = ev_ms(lambda: G.GPU_FFT_R(rbuf, temp, tt, cf, 2 * B), args.steps)
+
+        def rr():
+            G.GPU_FFT_R(rbuf, temp, tt, cf, 2 * B)
+            G.GPU_FFT_R(rbuf, temp, it, ci, B, True)
+        t_mul = ev_ms(rr, args.steps)
+        rbuf.copy_(rsrc)
+        rr()
+        torch.cuda.synchronize()
+        prod = np.round(rbuf[:B * N].cpu().numpy().reshape(B, N)).astype(np.int64)
+        exact = _poly_check(a, b, prod)
+        fwd_bytes = 2 * B * (N * 8 + N * 16)   # real in, full complex spectrum out
+        mul_bytes = B * 3 * N * 8              # two real operands in, real product out
+        print(json.dumps({"config": "R_R cyclic products, N=2^16 (degree < 2^15), batch 64, "
+                                    "[-2^10,2^10), f64",
+                          "forward_ms": t_fwd, "forward_alg_GBps": fwd_bytes / (t_fwd * 1e-3) / 1e9,
+                          "multiply_ms": t_mul, "multiply_alg_GBps": mul_bytes / (t_mul * 1e-3) / 1e9,
+                          "products_per_s": B / (t_mul * 1e-3),
+                          "bit_exact_at_random_points_mod_2^31-1": exact}), flush=True)
+        del rsrc, rbuf, temp
+
+        # ---- special FFT, l = 2^15 slots, batch 512
+        B, n = 512, 15
+        l = 1 << n
+        rng = np.random.default_rng(S.SEED_BASE % (1 << 32) + 5)
+        z = (np.sqrt(rng.random(B * l)) * np.exp(2j * np.pi * rng.random(B * l)) * 2.0 ** 30)
+        zs = torch.from_numpy(z).to(dev)
+        x = torch.empty_like(zs)
+        T64 = G.generate_table(G.TABLE_SPECIAL, n, False, stream=sid)
+        Ti64 = G.generate_table(G.TABLE_SPECIAL, n, True, stream=sid)
+        T32 = G.generate_table(G.TABLE_SPECIAL, n, False, prec="f32", stream=sid)
+        Ti32 = G.generate_table(G.TABLE_SPECIAL, n, True, prec="f32", stream=sid)
+        cfw = G.config(n, G.FORWARD, stream=sid)
+        cin = G.config(n, G.INVERSE, mod_inverse=1.0 / l, stream=sid)
+        work = torch.empty(2 * B * l, dtype=torch.complex64, device=dev)
+        x.copy_(zs)
+        t_enc64 = ev_ms(lambda: G.GPU_Special_FFT(x, Ti64, cin, B), args.steps)
+        t_dec64 = ev_ms(lambda: G.GPU_Special_FFT(x, T64, cfw, B), args.steps)
+        t_enc32 = ev_ms(lambda: G.special_fft_split(x, Ti32, cin, B, work), args.steps)
+        t_dec32 = ev_ms(lambda: G.special_fft_split(x, T32, cfw, B, work), args.steps)
+        x.copy_(zs)
+        G.GPU_Special_FFT(x, Ti64, cin, B)
+        enc64 = x.clone()
+        G.GPU_Special_FFT(x, T64, cfw, B)
+        dec64 = x.clone()
+        x.copy_(zs)
+        G.special_fft_split(x, Ti32, cin, B, work)
+        enc32 = x.clone()
+        G.special_fft_split(x, T32, cfw, B, work)
+        torch.cuda.synchronize()
+        e_enc = float((enc32 - enc64).abs().max() / enc64.abs().max())
+        e_rt = float((x - dec64).abs().max())
+        e_rt64 = float((dec64 - zs).abs().max())
+        bytes_tr = B * l * 16 * 2
+        print(json.dumps({"config": "special FFT, 2^15 slots x 2^30 on the unit disk, batch 512",
+                          "f64_encode_ms": t_enc64, "f64_decode_ms": t_dec64,
+                          "f64_alg_GBps": bytes_tr / (t_enc64 * 1e-3) / 1e9,
+                          "f32split_encode_ms": t_enc32, "f32split_decode_ms": t_dec32,
+                          "f32split_encode_rel_err_vs_f64": e_enc,
+                          "f32split_round_trip_max_slot_err": e_rt,
+                          "f64_round_trip_max_slot_err": e_rt64}), flush=True)
+
+
 def main():
     args = parse()
+    if args.configs:
+        config_suite(args)
+        return
     if args.calibrate_copy:
         calibrate_copy()
         return
