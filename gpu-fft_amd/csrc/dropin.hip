@@ -85,6 +85,26 @@ int r2r(void* real, void* temp, const void* table, const gpufft_config* cfg, int
         io.dst_stride = L;
         return run_transform<T>(MODE_STD_FWD, n, batch, nega, (const cplx<T>*)table, io, st, who);
     }
+    if (!nega && n >= 3) {
+        // X^N-1: half-length inverse (LD_RR_PACK), an (N/2)-point transform
+        // whose complex outputs are the real pairs (c_2m, c_2m+1); the real
+        // buffer holds the intermediate passes.  The real output of the
+        // reference is Re(mod_inverse * D_j) with D_j real for the spectrum
+        // of real data, i.e. Re(mod_inverse) * D_j.
+        const long long M = L / 2;
+        io.lk = LD_RR_PACK;
+        io.src = t;
+        io.src2 = mult ? t + (long long)batch * L : nullptr;
+        io.ltab = (const cplx<T>*)table;
+        io.src_stride = L;
+        io.sk = ST_CPLX;
+        io.dst = real;
+        io.dst_stride = M;
+        io.work = (cplx<T>*)real;
+        io.do_scale = true;
+        io.scale = mk<T>((T)cfg->mod_inverse_re, (T)0);
+        return run_transform<T>(MODE_STD_INV, n - 1, batch, 0, (const cplx<T>*)table, io, st, who);
+    }
     io.lk = mult ? LD_MUL : LD_CPLX;
     io.src = t;
     io.src2 = t + (long long)batch * L;

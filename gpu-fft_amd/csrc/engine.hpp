@@ -37,6 +37,7 @@ enum LoadKind {
     LD_FOLD = 3,
     LD_SPLIT_HI = 4,  // T = float: double complex source, its float rounding
     LD_SPLIT_LO = 5,  // T = float: double complex source, the float of the rest
+    LD_RR_PACK = 6,   // R_R inverse by half-length packing (see load_elem)
 };
 enum StoreKind {
     ST_CPLX = 0,
@@ -119,6 +120,23 @@ __device__ __forceinline__ cplx<T> load_elem(const PassArgs<T>& a, long long b, 
         const T* f = reinterpret_cast<const T*>(a.src) + b * a.src_stride;
         int L = 1 << a.n;
         return mk<T>(ld_s(f + p, nt), ld_s(f + p + L, nt)) * ldc(a.ltab + p);
+    } else if constexpr (LK == LD_RR_PACK) {
+        // Half-length form of the X^N-1 R_R inverse.  The source holds full
+        // N-point spectra C in bit-reversed order (C_k at 2 bitrev(k), C_{k+M}
+        // next to it, M = N/2 = 2^n); for a spectrum of real data the N-point
+        // inverse D_j = sum_k C_k w^-jk has D_2m + i D_2m+1 = (M-point inverse
+        // of Z)_m with Z_k = (C_k + C_{k+M}) + i w^-k (C_k - C_{k+M}), and
+        // w^-k = itab[p] at bit-reversed position p = bitrev(k).  src2
+        // (optional) is the second operand of the product.
+        const cplx<T>* X = reinterpret_cast<const cplx<T>*>(a.src) + b * a.src_stride + 2 * p;
+        cplx<T> x0 = ldc_s(X, nt), x1 = ldc_s(X + 1, nt);
+        if (a.src2) {
+            const cplx<T>* Y = reinterpret_cast<const cplx<T>*>(a.src2) + b * a.src_stride + 2 * p;
+            x0 = x0 * ldc_s(Y, nt);
+            x1 = x1 * ldc_s(Y + 1, nt);
+        }
+        const cplx<T> s0 = x0 + x1, d = (x0 - x1) * ldc(a.ltab + p);
+        return mk<T>(s0.x - d.y, s0.y + d.x);
     } else {  // LD_SPLIT_HI / LD_SPLIT_LO: x = hi + lo, hi = float(x), lo = float(x - hi)
         const cplx<double> x =
             ldc_s(reinterpret_cast<const cplx<double>*>(a.src) + b * a.src_stride + p, nt);

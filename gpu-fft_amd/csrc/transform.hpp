@@ -96,6 +96,7 @@ template <typename T> PassFn<T> select_pass(int mode, int lk, int sk)
     GFA_SEL(MODE_STD_INV, LD_CPLX, ST_REAL)
     GFA_SEL(MODE_STD_INV, LD_MUL, ST_REAL)
     GFA_SEL(MODE_STD_INV, LD_CPLX, ST_UNFOLD)
+    GFA_SEL(MODE_STD_INV, LD_RR_PACK, ST_CPLX)
     GFA_SEL(MODE_STD_INV, LD_MUL, ST_UNFOLD)
     GFA_SEL(MODE_SPE_FWD, LD_CPLX, ST_CPLX)
     GFA_SEL(MODE_SPE_INV, LD_CPLX, ST_CPLX)

@@ -94,6 +94,14 @@ def test_r2r_forward_inverse(prec, nega, n_power):
                 True)
     ref2 = O.r2r_inverse(ref, n_power, batch, tabs["inv_rev"], nega, 1, complex(1.0 / L), prec)
     assert rel_err(to_host(dr)[: batch * L], ref2) < TOL[prec]
+    # inverse without product, complex mod_inverse (the output is Re(mod_inverse * D)),
+    # all 2 batch spectra: the round trip returns the input scaled by Re(mod_inverse) L
+    dt3 = to_dev(ref)
+    mi = complex(1.0 / L, 0.37 / L)
+    G.GPU_FFT_R(dr, dt3, to_dev(tabs["inv_rev"]), G.config(n_power, G.INVERSE, rp, mi), 2 * batch)
+    ref3 = O.r2r_inverse(ref, n_power, 2 * batch, tabs["inv_rev"], nega, 0, mi, prec)
+    assert rel_err(to_host(dr), ref3) < TOL[prec]
+    assert rel_err(to_host(dr), real) < 10 * TOL[prec]
 
 
 @pytest.mark.parametrize("prec", PRECS)
