@@ -470,6 +470,18 @@ def config_suite(args):
                           "f64_round_trip_max_slot_err": e_rt64}), flush=True)
 
 
+def tune_radix(plan, candidates, fn, stream, steps):
+    """Radix bits of the product kernel: every candidate timed in three
+    interleaved rounds, the minimum kept (one round is noisy)."""
+    tune = {}
+    for _ in range(3):
+        for rbv in candidates:
+            plan.radix_bits = rbv
+            t = timed(fn, stream, max(5, steps // 2), 1)
+            tune[rbv] = min(tune.get(rbv, t), t)
+    return tune
+
+
 def main():
     args = parse()
     if args.configs:
@@ -519,11 +531,9 @@ def main():
         if "f64" in forms:
             plan = G.PolymulPlan(rp, G.PREC_F64, max_batch=batch, stream=sid)
             out = torch.empty(batch * N, dtype=torch.float64, device=dev)
-            tune = {}
-            for rbv in ([args.rb_f64] if args.rb_f64 else [3, 4]):
-                plan.radix_bits = rbv
-                tune[rbv] = timed(lambda: plan.multiply_f64(a64, b64, out, batch, stream=sid),
-                                  stream, max(3, args.steps // 4), 1)
+            tune = tune_radix(plan, [args.rb_f64] if args.rb_f64 else [3, 4],
+                              lambda: plan.multiply_f64(a64, b64, out, batch, stream=sid),
+                              stream, args.steps)
             plan.radix_bits = min(tune, key=tune.get)
             ms = timed(lambda: plan.multiply_f64(a64, b64, out, batch, stream=sid), stream,
                        args.steps, args.warmup)
@@ -547,11 +557,9 @@ def main():
             plan.a_center_log2 = args.a_center
             out = torch.empty(batch * N, dtype=torch.int64, device=dev)
             err = torch.empty(2 * batch, dtype=torch.float64, device=dev)
-            tune = {}
-            for rbv in ([args.rb_f32] if args.rb_f32 else [3, 4, 5]):
-                plan.radix_bits = rbv
-                tune[rbv] = timed(lambda: plan.multiply_i32(a32, b32, out, batch, stream=sid),
-                                  stream, max(3, args.steps // 4), 1)
+            tune = tune_radix(plan, [args.rb_f32] if args.rb_f32 else [3, 4, 5],
+                              lambda: plan.multiply_i32(a32, b32, out, batch, stream=sid),
+                              stream, args.steps)
             plan.radix_bits = min(tune, key=tune.get)
             ms = timed(lambda: plan.multiply_i32(a32, b32, out, batch, stream=sid), stream,
                        args.steps, args.warmup)

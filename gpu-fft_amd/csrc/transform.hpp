@@ -77,17 +77,18 @@ template <typename T> struct TransformIO {
 template <typename T>
 using PassFn = void (*)(PassArgs<T>);
 
-template <typename T, int MODE, int LK, int SK> inline PassFn<T> pass_fn()
+template <typename T, int MODE, int LK, int SK, int BIG> inline PassFn<T> pass_fn()
 {
-    return &pass_kernel<T, MODE, TileCfg<T>::LOGT, TileCfg<T>::E, LK, SK>;
+    return &pass_kernel<T, MODE, TileCfg<T>::LOGT + BIG, TileCfg<T>::E, LK, SK>;
 }
 
 // Kernel for (mode, load kind, store kind); nullptr for combinations no
-// operation uses.
-template <typename T> PassFn<T> select_pass(int mode, int lk, int sk)
+// operation uses.  BIG = 1: tile of 2^(LOGT+1) elements (twice the LDS and
+// threads), so that a transform of exactly 2^(LOGT+1) points is one pass.
+template <typename T, int BIG> PassFn<T> select_pass(int mode, int lk, int sk)
 {
 #define GFA_SEL(M, L, S) \
-    if (mode == M && lk == L && sk == S) return pass_fn<T, M, L, S>();
+    if (mode == M && lk == L && sk == S) return pass_fn<T, M, L, S, BIG>();
     GFA_SEL(MODE_STD_FWD, LD_CPLX, ST_CPLX)
     GFA_SEL(MODE_STD_FWD, LD_REAL, ST_CPLX)
     GFA_SEL(MODE_STD_FWD, LD_FOLD, ST_CPLX)
@@ -119,8 +120,13 @@ template <typename T>
 int run_transform(int mode, int n, int batch, int nega, const cplx<T>* tab,
                   const TransformIO<T>& io, hipStream_t stream, const char* who)
 {
-    constexpr int LOGT = TileCfg<T>::LOGT;
-    constexpr int NT = (1 << LOGT) >> TileCfg<T>::E;
+    constexpr int LOGT0 = TileCfg<T>::LOGT;
+    // double, n = LOGT0 + 1: one pass over a double-size tile (128 KiB of LDS,
+    // 512 threads) instead of two passes (for float the doubled tile would
+    // need 1024 threads, whose 128-register budget spills)
+    const bool big = sizeof(T) == 8 && n == LOGT0 + 1;
+    const int LOGT = big ? LOGT0 + 1 : LOGT0;
+    const int NT = (1 << LOGT) >> TileCfg<T>::E;
     std::vector<PassPlan> plan = plan_passes(n, LOGT, mode_descending(mode));
     const long long L = 1LL << n;
     for (size_t i = 0; i < plan.size(); i++) {
@@ -159,7 +165,9 @@ int run_transform(int mode, int n, int batch, int nega, const cplx<T>* tab,
             a.dst = io.work;
             a.dst_stride = L;
         }
-        PassFn<T> fn = select_pass<T>(mode, lk, sk);
+        PassFn<T> fn = select_pass<T, 0>(mode, lk, sk);
+        if constexpr (sizeof(T) == 8)
+            if (big) fn = select_pass<T, 1>(mode, lk, sk);
         if (!fn) return fail(GPUFFT_ERR_UNSUPPORTED, std::string(who) + ": no kernel for this pass");
         long long tiles = ((long long)batch << n) >> LOGT;
         if (pp.logBT > 0) tiles = ((long long)batch + (1LL << pp.logBT) - 1) >> pp.logBT;
