@@ -127,13 +127,18 @@ SPLITS = {
     "a4x6": (4, 6, 0, 1),
     "a7x3_b4x2": (7, 3, 4, 2),
     "a4x5c19": (4, 5, 0, 1, 19),
-    "a7x3c20_b4x2": (7, 3, 4, 2, 20),
+    "a6x4c19_b4x2": (6, 4, 4, 2, 19),
 }
+
+
+# splits at the edge of the FP32 margin at this ring size (a in [0, 2^20),
+# b in [-2^7, 2^7]): the kernel may report a largest distance of 0.5
+BEYOND_BOUND = {(17, "a5x5"), (17, "a7x3_b4x2")}
 
 
 @pytest.mark.parametrize("rb", [3, 4, 5])
 @pytest.mark.parametrize("split", sorted(SPLITS))
-@pytest.mark.parametrize("ring_power", [8, 12, 15, 16])
+@pytest.mark.parametrize("ring_power", [8, 12, 13, 14, 15, 16, 17])
 def test_polymul_f32_split_exact(ring_power, split, rb):
     G = _g()
     import torch
@@ -153,8 +158,14 @@ def test_polymul_f32_split_exact(ring_power, split, rb):
     got = to_host(out).reshape(batch, N)
     e = to_host(err).reshape(batch, 2)
     exact = O.ntt_negacyclic(a, b)
-    assert np.all(e[:, 0] < 0.5), e
-    assert np.array_equal(got, exact)
+    # the guarantee: a product whose largest distance to the rounded value is
+    # below 0.5 is exact; only splits at the edge of the FP32 margin may report
+    # 0.5 or more (then the product is flagged, not promised)
+    if (ring_power, split) not in BEYOND_BOUND:
+        assert np.all(e[:, 0] < 0.5), e
+    for i in range(batch):
+        if e[i, 0] < 0.5:
+            assert np.array_equal(got[i], exact[i]), i
 
 
 def test_polymul_f32_unsplit_small_coefficients():
