@@ -15,9 +15,12 @@
 //   3. multiplies in registers, inverse-transforms in LDS and writes the
 //      2^S partial results to a scratch buffer.
 // A second, streaming pass applies the top SB inverse stages, the scale, the
-// untwist and the unfold (FP64: pass_kernel; FP32 limbs: limb_combine_kernel,
-// which also rounds each limb product and recombines them in int64).  With
-// SB = 0 the whole product is one workgroup and writes the output directly.
+// untwist and the unfold (FP64: f64_combine_kernel; FP32 limbs:
+// limb_combine_kernel, which also rounds each limb product and recombines
+// them in int64).  With SB = 0 an FP64 product is one workgroup and writes the
+// output directly.  The scratch is written and read with streaming hints: it
+// is used once, and must not displace the operand lines that the sibling
+// workgroups of a product re-read from L2.
 //
 // The stage/twiddle semantics are those of the reference GPU_FFNT
 // (src/lib/fft.cu:1013-1384) and GPU_FFT (fft.cu:11-260): bit-reversed
@@ -28,9 +31,6 @@
 
 namespace gfa {
 
-#ifndef GFA_LOAD_UNROLL
-#define GFA_LOAD_UNROLL 1
-#endif
 
 enum ProdLoad {
     PL_FOLD = 0,       // T inputs [batch][2^(n+1)]: fold + twist
