@@ -2,23 +2,44 @@
 // split, int32 in / int64 out).  No counterpart in the reference, which runs
 // one GPU_FFT / GPU_FFNT call pair per ring size.
 //
-// Every product is cut into 2^SB sub-problems of 2^S points (S = min(log2 L,
-// 14), L the complex transform length) as in fused.hpp; the work list holds
-// one item per sub-problem, largest products first.  A persistent grid of
-// workgroups takes items from a device counter until the list is exhausted
-// (every workgroup reaches that exit).  An item with SB = 0 is a whole
-// product: it rounds each limb product and accumulates it into the output.
-// An item with SB > 0 writes its limb sub-products to scratch, publishes them
-// (agent-scope release), and increments the product's arrival counter; the
-// workgroup that arrives last (acquire) runs the top SB inverse stages, the
-// output transform and the rounding for the whole product.  Nothing waits
-// on another workgroup.
+// Both product kinds are FFNT products (fold + twist, L complex points;
+// reference GPU_FFNT, src/lib/fft.cu:1013-1384):
+//   NEGACYCLIC  c = a b mod X^N + 1, ring N, L = N/2.
+//   LINEAR      c = a b (2N - 1 coefficients, then a 0), computed as the
+//               negacyclic product of ring 2N of the zero-padded operands:
+//               deg(a b) < 2N, so nothing wraps.  L = N, and the fold's upper
+//               half is the padding, so the imaginary inputs are 0.  This
+//               stands for the reference's zero-padded cyclic C_C product of
+//               length 2N (example/test_fft_gpu_C_C.cu) at half its
+//               transform length.
 //
-// Product kinds:
-//   NEGACYCLIC  c = a b mod X^N + 1 (FFNT: L = N/2, fold + twist; reference
-//               GPU_FFNT, src/lib/fft.cu:1013-1384)
-//   LINEAR      c = a b, 2N - 1 coefficients (zero-padded cyclic C_C product
-//               of length L = 2N; reference example/test_fft_gpu_C_C.cu)
+// Every product is cut into 2^SB sub-problems of 2^S points (S = min(log2 L,
+// 14)) as in fused.hpp; the work list holds one item per sub-problem.  The
+// list is split into 8 queues, one per XCD: the hardware deals the workgroups
+// of the persistent grid round-robin to the 8 XCDs, and workgroup b serves
+// queue b mod 8 first.  All sub-problems of a product sit in one queue, so
+// the workgroups that re-read its operands share an L2.  Products are dealt
+// to the queues largest first, each to the least loaded queue.  A workgroup
+// whose queue is exhausted serves the other queues in turn and exits once all
+// 8 are exhausted (every counter only grows, so every workgroup gets there).
+//
+// Limb products: each limb of a and of b is gathered and forward-transformed
+// once per item.  Limb products of equal weight 2^(wa ia + wb jb) (with
+// wa = wb: a0 b1 and a1 b0, ...) are summed in the frequency domain and share
+// one inverse transform; their magnitudes add, which the error statistics
+// see.  Spectra and partial sums that do not stay in registers live in a
+// small per-workgroup stash.  For a and b in 2 limbs each that is 4 forward
+// and 3 inverse transforms instead of 4 products of 2 forward and 1 inverse.
+//
+// An item with SB = 0 is a whole product: it rounds each group and
+// accumulates it into the output.  An item with SB > 0 writes its group
+// sub-products to scratch.  The top SB inverse stages, the rounding and the
+// recombination of a split product then run in one of two places:
+//   * a second kernel over all split products (the default);
+//   * the persistent kernel itself: each sub-problem publishes its scratch
+//     (agent-scope release) and increments the product's arrival counter, and
+//     the workgroup that arrives last (acquire) combines the product.  Nothing
+//     waits on another workgroup.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,15 +59,17 @@ constexpr int KIND_NEGA = 0;
 constexpr int KIND_LINEAR = 1;
 constexpr int BT = 512;     // threads of the persistent workgroup
 constexpr int SMAX = 14;    // 2^14 float complex = 128 KiB of LDS
-constexpr int SMIN = 6;
+constexpr int SMIN = 5;
 constexpr int SBMAX = 4;
+constexpr int NQ = 8;       // work queues, one per XCD
+constexpr int CT = 256;     // threads of a combine workgroup
 
 struct BProd {
     int kind;
     int n;        // log2 L
     int S, SB;
     int N;        // coefficients per operand
-    int tabset;   // index of the table set for (kind, n)
+    int tabset;   // index of the table set for n
     long long a_off, b_off, out_off;
     long long scratch_off;  // cplx<float> offset (SB > 0)
 };
@@ -54,21 +77,43 @@ struct BProd {
 struct BTables {
     const cplx<float>* root;
     const cplx<float>* iroot;
-    const cplx<float>* twist;    // NEGACYCLIC only
-    const cplx<float>* untwist;  // NEGACYCLIC only
+    const cplx<float>* twist;
+    const cplx<float>* untwist;
+};
+
+// Order of the limb products of one product.  Limb products of equal
+// weight wa ia + wb jb form a group: they are summed in the frequency domain
+// and share one inverse transform.  Index k = ia kb + jb.
+constexpr int MAXP = 64;
+struct BSched {
+    int G;               // groups = inverse transforms per product
+    int gshift[MAXP];    // weight exponent of group g
+    int gslot[MAXP];     // stash slot of group g's partial sum (-1: none)
+    int gdone[MAXP];     // a limb after which group g is complete
+    int cgroup[MAXP];    // group of limb product k
+    int cfirst[MAXP];    // 1: k is the first contribution to its group's slot
+    int dgroup[MAXP];    // per a limb: group summed in registers and inverted
+                         // right after it (-1: none)
+    int dload[MAXP];     // per a limb: 1 if that group has a partial sum in its slot
 };
 
 struct BArgs {
     const BProd* prods;
-    const int2* items;  // (product, q)
+    const int2* items;  // (product, q), queue after queue
     const BTables* tabs;
-    int n_items;
-    int* next_item;     // work counter
-    int* arrive;        // per product
+    int qbeg[NQ];       // first item of each queue
+    int qlen[NQ];       // items of each queue
+    int* next_item;     // NQ queue counters
+    int* arrive;        // per product (in-kernel combine)
+    const int* split;   // the products with SB > 0 (combine kernel)
+    int cmb;            // 1: split products are combined in the persistent kernel
     const int* a;
     const int* b;
     long long* out;
     cplx<float>* scratch;
+    const BSched* sched;
+    cplx<float>* stash;  // stash_slots x 2^SMAX per workgroup of the persistent grid
+    int stash_slots;     // b-limb spectra 1 .. kb-1, then group partial sums
     const cplx<float>* wc;
     const cplx<float>* wci;
     int wa, ka, wb, kb;
@@ -88,6 +133,8 @@ __global__ void zero_doubles(double* p, int n)
     if (i < n) p[i] = 0.0;
 }
 
+// Largest e over the wave, folded into err[prod]; every lane of the wave
+// takes part.
 __device__ __forceinline__ void err_update(double* err, int prod, float e)
 {
     if (!err) return;
@@ -99,57 +146,205 @@ __device__ __forceinline__ void err_update(double* err, int prod, float e)
                   (unsigned long long)__double_as_longlong(d));
 }
 
-// Input at position P of the complex transform of one operand limb.
-template <int KIND>
-__device__ __forceinline__ void item_input(const int* x, int N, int L, int w, int limb, int k,
-                                           long long o, int P, float& re, float& im)
+// Global-memory views of the pointers the item code works through: the
+// pointers reach it inside structures, where the compiler no longer knows
+// their address space and would use flat instructions (which also count
+// against the LDS wait counter).
+// (The host pass of the compiler parses this code too but never runs it; it
+// gets plain pointers.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T> using gptr = __attribute__((address_space(1))) T*;
+#else
+template <typename T> using gptr = T*;
+#endif
+template <typename T> __device__ __forceinline__ gptr<T> to_global(T* p)
 {
-    if constexpr (KIND == KIND_NEGA) {
-        re = (float)limb_digit(x[P], w, limb, k, o);
-        im = (float)limb_digit(x[P + L], w, limb, k, o);
-    } else {
-        re = P < N ? (float)limb_digit(x[P], w, limb, k, o) : 0.f;
-        im = 0.f;
-    }
+    return (gptr<T>)p;
 }
 
-template <int KIND, int S, int SB, int E>
-__device__ __forceinline__ void item_load(cplx<float>* lds, const ProdArgs<float>& pa, const int* x,
-                                          int N, int w, int limb, int k, long long o,
-                                          const cplx<float>* C, int tid)
+// Sub-problem q of one limb of one operand into LDS: for each of the
+// workgroup's 2^S positions c, row q of the top SB forward stages over the
+// folded inputs at j 2^S + c (Q = 2^SB partners), times the twist
+// (coefficients C as in top_row_coeffs).  LINEAR products have no upper fold
+// half.  U positions per round, their loads issued together.
+template <int KIND, int S, int Q, int E>
+__device__ __forceinline__ void item_load(cplx<float>* lds, gptr<const cplx<float>> twist,
+                                          gptr<const int> x, int L, int w, int limb, int k,
+                                          long long o, const cplx<float>* C, int tid)
 {
     constexpr int NT = (1 << S) >> E;
-    constexpr int Q = 1 << SB;
-    const int L = 1 << pa.n;
+    constexpr int EPT = 1 << E;
+    constexpr int U0 = Q >= 4 ? 2 : (Q == 2 ? 4 : 8);
+    constexpr int U = U0 < EPT ? U0 : EPT;
     if (tid >= NT) return;
-#pragma unroll 2
-    for (int i = 0; i < (1 << E); i++) {
-        const int c = tid + i * NT;
-        float ax = 0.f, ay = 0.f;
+    for (int i0 = 0; i0 < EPT; i0 += U) {
+        int xr[U][Q], xi[U][Q];
+        cplx<float> tw[U];
 #pragma unroll
-        for (int j = 0; j < Q; j++) {
-            float re, im;
-            item_input<KIND>(x, N, L, w, limb, k, o, (j << S) | c, re, im);
-            ax = ax + C[j].x * re - C[j].y * im;
-            ay = ay + C[j].x * im + C[j].y * re;
+        for (int u = 0; u < U; u++) {
+            const int c = tid + (i0 + u) * NT;
+            tw[u] = twist[c];
+#pragma unroll
+            for (int j = 0; j < Q; j++) {
+                const int P = (j << S) | c;
+                xr[u][j] = x[P];
+                if constexpr (KIND == KIND_NEGA) xi[u][j] = x[P + L];
+            }
         }
-        cplx<float> y = mk<float>(ax, ay);
-        if constexpr (KIND == KIND_NEGA) y = pa.twist[c] * y;
-        lds[swz(c)] = y;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int c = tid + (i0 + u) * NT;
+            float ax = 0.f, ay = 0.f;
+#pragma unroll
+            for (int j = 0; j < Q; j++) {
+                const float re = (float)limb_digit(xr[u][j], w, limb, k, o);
+                if constexpr (KIND == KIND_NEGA) {
+                    const float im = (float)limb_digit(xi[u][j], w, limb, k, o);
+                    ax = ax + C[j].x * re - C[j].y * im;
+                    ay = ay + C[j].x * im + C[j].y * re;
+                } else {
+                    ax = ax + C[j].x * re;
+                    ay = ay + C[j].y * re;
+                }
+            }
+            lds[swz(c)] = tw[u] * mk<float>(ax, ay);
+        }
     }
 }
 
-// Whole product or one sub-problem of it.
-template <int KIND, int S, int SB, int E, int RB>
-__device__ void run_item(cplx<float>* lds, const BArgs& ba, const BProd& pr, const BTables& tb,
-                         int q, int tid, int* sh_last)
+// Streaming load and store of one element through a global pointer.
+__device__ __forceinline__ cplx<float> ldg_nt(gptr<const cplx<float>> p)
+{
+    typedef float vec2 __attribute__((ext_vector_type(2)));
+    const vec2 w = __builtin_nontemporal_load((gptr<const vec2>)p);
+    return mk<float>(w.x, w.y);
+}
+
+__device__ __forceinline__ void stg_nt(gptr<cplx<float>> p, cplx<float> v)
+{
+    typedef float vec2 __attribute__((ext_vector_type(2)));
+    vec2 w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, (gptr<vec2>)p);
+}
+
+// The top SB inverse stages of position c < 2^SMAX of a split product over
+// all its limb-group sub-products in scratch, then the scale 1/L, the
+// untwist, the rounding of each group and the int64 recombination; writes
+// the unfolded outputs out[P], out[P + L] for P = j 2^SMAX + c.  Returns the
+// largest distance of a group output to its rounding.
+template <int SB>
+__device__ __forceinline__ float combine_pos(const BArgs& ba, const BProd& pr, const BTables& tb,
+                                             int c)
+{
+    constexpr int Q = 1 << SB;
+    constexpr int S = SMAX;
+    const int L = 1 << pr.n;
+    const float scale = 1.0f / (float)L;
+    const int ngrp = ba.sched->G;
+    long long acc[Q][2];
+#pragma unroll
+    for (int j = 0; j < Q; j++) acc[j][0] = acc[j][1] = 0;
+    float em = 0.f;
+    const gptr<const cplx<float>> untwist = to_global(tb.untwist);
+    for (int l = 0; l < ngrp; l++) {
+        const gptr<const cplx<float>> src = to_global(ba.scratch + pr.scratch_off + (long long)l * L);
+        cplx<float> y[Q];
+#pragma unroll
+        for (int j = 0; j < Q; j++) y[j] = ldg_nt(src + (j << S) + c);
+#pragma unroll
+        for (int i = 0; i < SB; i++) {  // Gentleman-Sande, bits S .. n-1
+#pragma unroll
+            for (int j0 = 0; j0 < Q; j0++) {
+                if (j0 & (1 << i)) continue;
+                const int j1 = j0 | (1 << i);
+                const cplx<float> w = tb.iroot[j0 >> (i + 1)];
+                const cplx<float> U = y[j0], V = y[j1];
+                y[j0] = U + V;
+                y[j1] = (U - V) * w;
+            }
+        }
+        const long long wgt = 1LL << ba.sched->gshift[l];
+#pragma unroll
+        for (int j = 0; j < Q; j++) {
+            const int P = (j << S) | c;
+            const cplx<float> z = mk<float>(y[j].x * scale, y[j].y * scale) * untwist[P];
+            const float r0 = rintf(z.x), r1 = rintf(z.y);
+            em = fmaxf(em, fmaxf(fabsf(z.x - r0), fabsf(z.y - r1)));
+            acc[j][0] += (long long)r0 * wgt;
+            acc[j][1] += (long long)r1 * wgt;
+        }
+    }
+    const gptr<long long> o = to_global(ba.out + pr.out_off);
+#pragma unroll
+    for (int j = 0; j < Q; j++) {
+        const int P = (j << S) | c;
+        __builtin_nontemporal_store(acc[j][0], o + P);
+        __builtin_nontemporal_store(acc[j][1], o + P + L);
+    }
+    return em;
+}
+
+__device__ __forceinline__ float combine_pos_rt(const BArgs& ba, const BProd& pr,
+                                                const BTables& tb, int c)
+{
+    switch (pr.SB) {
+        case 1: return combine_pos<1>(ba, pr, tb, c);
+        case 2: return combine_pos<2>(ba, pr, tb, c);
+        case 3: return combine_pos<3>(ba, pr, tb, c);
+        case 4: return combine_pos<4>(ba, pr, tb, c);
+        default: return 0.f;
+    }
+}
+
+// Copy of a workgroup-uniform value held in scalar registers.
+template <typename V> __device__ __forceinline__ V uniform_copy(const V& x)
+{
+    static_assert(sizeof(V) % 4 == 0, "whole dwords");
+    unsigned w[sizeof(V) / 4];
+    __builtin_memcpy(w, &x, sizeof(V));
+#pragma unroll
+    for (unsigned i = 0; i < sizeof(V) / 4; i++) w[i] = __builtin_amdgcn_readfirstlane(w[i]);
+    V r;
+    __builtin_memcpy(&r, w, sizeof(V));
+    return r;
+}
+
+// Stash element e (index in the thread's register layout) of slot s, for
+// thread t: for a fixed e consecutive threads touch consecutive elements.
+// Each thread only reads back what it wrote itself.
+template <int NT>
+__device__ __forceinline__ gptr<cplx<float>> stash_at(gptr<cplx<float>> base, int s, int e, int t)
+{
+    return base + ((long long)s << SMAX) + e * NT + t;
+}
+
+// Whole product or one sub-problem of it, in three phases:
+//   1. each b limb: gather, forward steps; the spectrum of limb 0 stays in
+//      registers, the others go to the workgroup's stash;
+//   2. each a limb: gather, forward steps, then in registers the products
+//      with every b-limb spectrum, summed by weight group (sc.cgroup): the
+//      group that completes with this limb (sc.dgroup) is summed in registers
+//      and inverse-transformed at once, the others are summed in stash slots;
+//   3. after a limb, every other group it completed is inverse-transformed
+//      from its slot.
+// An inverse transform ends in emit(): rounding and accumulation into the
+// output (SB = 0) or the store of the sub-result to scratch (SB > 0).
+template <int KIND, int S, int E, int RB>
+__device__ __forceinline__ void run_item(cplx<float>* lds, const BArgs& ba, const BProd& pr,
+                                         const BTables& tb, int prod, int q, int tid, int* sh_last)
 {
     constexpr int EPT = 1 << E;
     constexpr int NT = (1 << S) >> E;
-    constexpr int Q = 1 << SB;
+    constexpr int QMAX = S == SMAX ? 1 << SBMAX : 1;  // only S = SMAX items are split
+    const int SB = S == SMAX ? pr.SB : 0;
+    const int Q = 1 << SB;
     constexpr int NST = (S + RB - 1) / RB;
+    constexpr int NG = 1 << (E - RB);  // register groups of a thread
     const bool active = tid < NT;
     const int L = 1 << pr.n;
+    const BSched& sc = *ba.sched;
 
     ProdArgs<float> pa{};
     pa.n = pr.n;
@@ -173,62 +368,203 @@ __device__ void run_item(cplx<float>* lds, const BArgs& ba, const BProd& pr, con
     TileGeo gi = g;
     gi.tab = make_rsrc(tb.iroot);
 
-    // coefficients of row q of the top network (twist factored for FFNT)
-    cplx<float> C[Q];
-    {
-        cplx<float> wq[SB > 0 ? SB : 1];
-        top_row_twiddles<float, SB>(pa, q, wq);
-#pragma unroll
-        for (int j = 0; j < Q; j++) {
-            cplx<float> e[Q];
-#pragma unroll
-            for (int m = 0; m < Q; m++) e[m] = mk<float>(0.f, 0.f);
-            e[j] = KIND == KIND_NEGA ? tb.twist[j << S] : mk<float>(1.f, 0.f);
-            C[j] = top_forward_row<float, SB>(e, wq);
+    // coefficients of row q of the top network, twist factored
+    cplx<float> C[QMAX];
+    if constexpr (QMAX == 1) {
+        top_row_coeffs<float, 0>(pa, q, C);
+    } else {
+        switch (SB) {
+            case 0: top_row_coeffs<float, 0>(pa, q, C); break;
+            case 1: top_row_coeffs<float, 1>(pa, q, C); break;
+            case 2: top_row_coeffs<float, 2>(pa, q, C); break;
+            case 3: top_row_coeffs<float, 3>(pa, q, C); break;
+            default: top_row_coeffs<float, 4>(pa, q, C); break;
         }
     }
-    const int* xa = ba.a + pr.a_off;
-    const int* xb = ba.b + pr.b_off;
-    const int nprod = ba.ka * ba.kb;
+    const gptr<const int> xa = to_global(ba.a + pr.a_off);
+    const gptr<const int> xb = to_global(ba.b + pr.b_off);
+    const gptr<const cplx<float>> twist = to_global(tb.twist);
+    const gptr<const cplx<float>> untwist = to_global(tb.untwist);
+    const gptr<cplx<float>> stash =
+        to_global(ba.stash + (((long long)blockIdx.x * ba.stash_slots) << SMAX));
     const float scale = 1.0f / (float)L;
     float emax = 0.f;
+    bool first_out = true;
 
+    // gather with a compile-time partner count
+    auto gather = [&](gptr<const int> x, int w, int limb, int k, long long o, int t) {
+        if constexpr (QMAX == 1) {
+            item_load<KIND, S, 1, E>(lds, twist, x, L, w, limb, k, o, C, t);
+        } else {
+            switch (SB) {
+                case 0: item_load<KIND, S, 1, E>(lds, twist, x, L, w, limb, k, o, C, t); break;
+                case 1: item_load<KIND, S, 2, E>(lds, twist, x, L, w, limb, k, o, C, t); break;
+                case 2: item_load<KIND, S, 4, E>(lds, twist, x, L, w, limb, k, o, C, t); break;
+                case 3: item_load<KIND, S, 8, E>(lds, twist, x, L, w, limb, k, o, C, t); break;
+                default: item_load<KIND, S, 16, E>(lds, twist, x, L, w, limb, k, o, C, t); break;
+            }
+        }
+    };
+
+    // the inverse of group grp is in LDS
+    auto emit = [&](int grp, int u) {
+        if (SB == 0) {
+            if (active) {
+                const long long wgt = 1LL << __builtin_amdgcn_readfirstlane(sc.gshift[grp]);
+                const gptr<long long> o = to_global(ba.out + pr.out_off);
+#pragma unroll 4
+                for (int i = 0; i < EPT; i++) {
+                    const int c = u + i * NT;
+                    cplx<float> z = lds[swz(c)];
+                    z = mk<float>(z.x * scale, z.y * scale) * untwist[c];
+                    const float r0 = rintf(z.x), r1 = rintf(z.y);
+                    emax = fmaxf(emax, fmaxf(fabsf(z.x - r0), fabsf(z.y - r1)));
+                    const long long v0 = (long long)r0 * wgt, v1 = (long long)r1 * wgt;
+                    if (first_out) {
+                        o[c] = v0;
+                        o[c + L] = v1;
+                    } else {
+                        o[c] += v0;
+                        o[c + L] += v1;
+                    }
+                }
+            }
+            first_out = false;
+        } else {
+            if (active) {
+                const gptr<cplx<float>> dst = to_global(ba.scratch + pr.scratch_off +
+                                                        (long long)grp * L + ((long long)q << S));
+#pragma unroll 4
+                for (int i = 0; i < EPT; i++) {
+                    const int c = u + i * NT;
+                    stg_nt(dst + c, lds[swz(c)]);
+                }
+            }
+        }
+    };
+
+    // phase 1: b-limb spectra
     cplx<float> breg[EPT];
     for (int jb = 0; jb < ba.kb; jb++) {
         int t = tid;
         asm volatile("" : "+v"(t));
         __syncthreads();
-        item_load<KIND, S, SB, E>(lds, pa, xb, pr.N, ba.wb, jb, ba.kb, ba.ob, C, t);
+        gather(xb, ba.wb, jb, ba.kb, ba.ob, t);
         run_steps_ct_active<float, MODE_STD_FWD, E, NT, S, RB, 0, NST - 1>(lds, pf, g, t, ba.wc, active);
         if (active) {
 #pragma unroll
-            for (int kk = 0; kk < (1 << (E - RB)); kk++) {
+            for (int kk = 0; kk < NG; kk++) {
                 const int rbase = (t + kk * NT) << RB;
                 cplx<float> v[1 << RB];
 #pragma unroll
                 for (int j = 0; j < (1 << RB); j++) v[j] = lds[swz(rbase | j)];
                 group_stages_derived<float, MODE_STD_FWD, RB>(v, pf, g, rbase, 0, 0, ba.wc);
 #pragma unroll
-                for (int j = 0; j < (1 << RB); j++) breg[(kk << RB) + j] = v[j];
+                for (int j = 0; j < (1 << RB); j++) {
+                    const int e = (kk << RB) + j;
+                    if (jb == 0)
+                        breg[e] = v[j];
+                    else
+                        *stash_at<NT>(stash, jb - 1, e, t) = v[j];
+                }
             }
         }
-        for (int ia = 0; ia < ba.ka; ia++) {
-            int u = tid;
-            asm volatile("" : "+v"(u));
+    }
+
+    // phases 2 and 3
+    for (int ia = 0; ia < ba.ka; ia++) {
+        int u = tid;
+        asm volatile("" : "+v"(u));
+        __syncthreads();
+        gather(xa, ba.wa, ia, ba.ka, ba.oa, u);
+        run_steps_ct_active<float, MODE_STD_FWD, E, NT, S, RB, 0, NST - 1>(lds, pf, g, u, ba.wc,
+                                                                       active);
+        const int dg = __builtin_amdgcn_readfirstlane(sc.dgroup[ia]);
+        const bool dl = __builtin_amdgcn_readfirstlane(sc.dload[ia]) != 0;
+        const int dslot = dl ? __builtin_amdgcn_readfirstlane(sc.gslot[dg]) : 0;
+        if (active) {
+#pragma unroll
+            for (int kk = 0; kk < NG; kk++) {
+                const int rbase = (u + kk * NT) << RB;
+                cplx<float> v[1 << RB];
+#pragma unroll
+                for (int j = 0; j < (1 << RB); j++) v[j] = lds[swz(rbase | j)];
+                group_stages_derived<float, MODE_STD_FWD, RB>(v, pf, g, rbase, 0, 0, ba.wc);
+                // products with the b-limb spectra, CH elements at a time;
+                // each round issues all its stash loads before its stores
+                constexpr int CH = (1 << RB) < 8 ? (1 << RB) : 8;
+#pragma unroll
+                for (int j0 = 0; j0 < (1 << RB); j0 += CH) {
+                    const int e0 = (kk << RB) + j0;
+                    cplx<float> w[CH];
+                    if (dl) {
+                        const gptr<cplx<float>> ds = stash_at<NT>(stash, dslot, e0, u);
+#pragma unroll
+                        for (int jj = 0; jj < CH; jj++) w[jj] = ds[jj * NT];
+                    } else {
+#pragma unroll
+                        for (int jj = 0; jj < CH; jj++) w[jj] = mk<float>(0.f, 0.f);
+                    }
+                    for (int jb = 0; jb < ba.kb; jb++) {
+                        const int k = ia * ba.kb + jb;
+                        const int cg = __builtin_amdgcn_readfirstlane(sc.cgroup[k]);
+                        const bool regs = cg == dg;
+                        const bool first = __builtin_amdgcn_readfirstlane(sc.cfirst[k]) != 0;
+                        const int cslot = regs ? 0 : __builtin_amdgcn_readfirstlane(sc.gslot[cg]);
+                        const gptr<cplx<float>> slot = stash_at<NT>(stash, cslot, e0, u);
+                        cplx<float> bj[CH], c[CH];
+                        if (jb == 0) {
+#pragma unroll
+                            for (int jj = 0; jj < CH; jj++) bj[jj] = breg[e0 + jj];
+                        } else {
+                            const gptr<cplx<float>> bsp = stash_at<NT>(stash, jb - 1, e0, u);
+#pragma unroll
+                            for (int jj = 0; jj < CH; jj++) bj[jj] = bsp[jj * NT];
+                        }
+#pragma unroll
+                        for (int jj = 0; jj < CH; jj++) c[jj] = v[j0 + jj] * bj[jj];
+                        if (regs) {
+#pragma unroll
+                            for (int jj = 0; jj < CH; jj++) w[jj] = w[jj] + c[jj];
+                        } else if (first) {
+#pragma unroll
+                            for (int jj = 0; jj < CH; jj++) slot[jj * NT] = c[jj];
+                        } else {
+                            cplx<float> old[CH];
+#pragma unroll
+                            for (int jj = 0; jj < CH; jj++) old[jj] = slot[jj * NT];
+#pragma unroll
+                            for (int jj = 0; jj < CH; jj++) slot[jj * NT] = old[jj] + c[jj];
+                        }
+                    }
+#pragma unroll
+                    for (int jj = 0; jj < CH; jj++) v[j0 + jj] = w[jj];
+                }
+                if (dg >= 0) {
+                    group_stages_derived<float, MODE_STD_INV, RB>(v, pinv, gi, rbase, 0, 0, ba.wci);
+#pragma unroll
+                    for (int j = 0; j < (1 << RB); j++) lds[swz(rbase | j)] = v[j];
+                }
+            }
+        }
+        if (dg >= 0) {
+            run_steps_ct_active<float, MODE_STD_INV, E, NT, S, RB, 1, NST>(lds, pinv, gi, u, ba.wci,
+                                                                         active);
+            emit(dg, u);
+        }
+        const int ngrp = __builtin_amdgcn_readfirstlane(sc.G);
+        for (int grp = 0; grp < ngrp; grp++) {
+            if (__builtin_amdgcn_readfirstlane(sc.gdone[grp]) != ia || grp == dg) continue;
+            const int gs = __builtin_amdgcn_readfirstlane(sc.gslot[grp]);
             __syncthreads();
-            item_load<KIND, S, SB, E>(lds, pa, xa, pr.N, ba.wa, ia, ba.ka, ba.oa, C, u);
-            run_steps_ct_active<float, MODE_STD_FWD, E, NT, S, RB, 0, NST - 1>(lds, pf, g, u, ba.wc,
-                                                                           active);
             if (active) {
 #pragma unroll
-                for (int kk = 0; kk < (1 << (E - RB)); kk++) {
+                for (int kk = 0; kk < NG; kk++) {
                     const int rbase = (u + kk * NT) << RB;
                     cplx<float> v[1 << RB];
 #pragma unroll
-                    for (int j = 0; j < (1 << RB); j++) v[j] = lds[swz(rbase | j)];
-                    group_stages_derived<float, MODE_STD_FWD, RB>(v, pf, g, rbase, 0, 0, ba.wc);
-#pragma unroll
-                    for (int j = 0; j < (1 << RB); j++) v[j] = v[j] * breg[(kk << RB) + j];
+                    for (int j = 0; j < (1 << RB); j++)
+                        v[j] = *stash_at<NT>(stash, gs, (kk << RB) + j, u);
                     group_stages_derived<float, MODE_STD_INV, RB>(v, pinv, gi, rbase, 0, 0, ba.wci);
 #pragma unroll
                     for (int j = 0; j < (1 << RB); j++) lds[swz(rbase | j)] = v[j];
@@ -236,59 +572,17 @@ __device__ void run_item(cplx<float>* lds, const BArgs& ba, const BProd& pr, con
             }
             run_steps_ct_active<float, MODE_STD_INV, E, NT, S, RB, 1, NST>(lds, pinv, gi, u, ba.wci,
                                                                          active);
-            const int limb = ia * ba.kb + jb;
-            if constexpr (SB == 0) {
-                // the whole product: round this limb product and accumulate
-                if (active) {
-                    const long long wgt = 1LL << (ba.wa * ia + ba.wb * jb);
-                    long long* o = ba.out + pr.out_off;
-#pragma unroll 4
-                    for (int i = 0; i < EPT; i++) {
-                        const int c = u + i * NT;
-                        cplx<float> z = lds[swz(c)];
-                        z = mk<float>(z.x * scale, z.y * scale);
-                        if constexpr (KIND == KIND_NEGA) {
-                            z = z * tb.untwist[c];
-                            const float r0 = rintf(z.x), r1 = rintf(z.y);
-                            emax = fmaxf(emax, fmaxf(fabsf(z.x - r0), fabsf(z.y - r1)));
-                            const long long v0 = (long long)r0 * wgt, v1 = (long long)r1 * wgt;
-                            if (limb == 0) {
-                                o[c] = v0;
-                                o[c + L] = v1;
-                            } else {
-                                o[c] += v0;
-                                o[c + L] += v1;
-                            }
-                        } else {
-                            const float r0 = rintf(z.x);
-                            emax = fmaxf(emax, fabsf(z.x - r0));
-                            const long long v0 = (long long)r0 * wgt;
-                            o[c] = limb == 0 ? v0 : o[c] + v0;
-                        }
-                    }
-                }
-            } else {
-                if (active) {
-                    cplx<float>* dst = ba.scratch + pr.scratch_off + (long long)limb * L +
-                                       ((long long)q << S);
-#pragma unroll 4
-                    for (int i = 0; i < EPT; i++) {
-                        const int c = u + i * NT;
-                        stc_nt(dst + c, lds[swz(c)]);
-                    }
-                }
-            }
+            emit(grp, u);
         }
     }
-    if constexpr (SB == 0) {
-        err_update(ba.err, (int)(&pr - ba.prods), emax);
-    } else {
+    if (SB == 0) {
+        err_update(ba.err, prod, emax);
+    } else if (ba.cmb) {
         // publish this sub-problem; the last of the product's Q workgroups
         // combines them
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __syncthreads();
         if (tid == 0) {
-            const int prod = (int)(&pr - ba.prods);
             const int prev = __hip_atomic_fetch_add(ba.arrive + prod, 1, __ATOMIC_ACQ_REL,
                                                     __HIP_MEMORY_SCOPE_AGENT);
             *sh_last = (prev == Q - 1) ? 1 : 0;
@@ -296,93 +590,58 @@ __device__ void run_item(cplx<float>* lds, const BArgs& ba, const BProd& pr, con
         __syncthreads();
         if (*sh_last) {
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            long long* o = ba.out + pr.out_off;
             float em = 0.f;
-            for (int c = tid; c < (1 << S); c += BT) {
-                long long acc[Q][2];
-#pragma unroll
-                for (int j = 0; j < Q; j++) acc[j][0] = acc[j][1] = 0;
-                for (int l = 0; l < nprod; l++) {
-                    const cplx<float>* src = ba.scratch + pr.scratch_off + (long long)l * L;
-                    cplx<float> y[Q];
-#pragma unroll
-                    for (int j = 0; j < Q; j++) y[j] = src[(j << S) + c];
-#pragma unroll
-                    for (int i = 0; i < SB; i++) {  // Gentleman-Sande, bits S .. n-1
-#pragma unroll
-                        for (int j0 = 0; j0 < Q; j0++) {
-                            if (j0 & (1 << i)) continue;
-                            const int j1 = j0 | (1 << i);
-                            const cplx<float> w = tb.iroot[j0 >> (i + 1)];
-                            cplx<float> U = y[j0], V = y[j1];
-                            y[j0] = U + V;
-                            y[j1] = (U - V) * w;
-                        }
-                    }
-                    const long long wgt = 1LL << (ba.wa * (l / ba.kb) + ba.wb * (l % ba.kb));
-#pragma unroll
-                    for (int j = 0; j < Q; j++) {
-                        const int P = (j << S) | c;
-                        cplx<float> z = mk<float>(y[j].x * scale, y[j].y * scale);
-                        if constexpr (KIND == KIND_NEGA) {
-                            z = z * tb.untwist[P];
-                            const float r0 = rintf(z.x), r1 = rintf(z.y);
-                            em = fmaxf(em, fmaxf(fabsf(z.x - r0), fabsf(z.y - r1)));
-                            acc[j][0] += (long long)r0 * wgt;
-                            acc[j][1] += (long long)r1 * wgt;
-                        } else {
-                            const float r0 = rintf(z.x);
-                            em = fmaxf(em, fabsf(z.x - r0));
-                            acc[j][0] += (long long)r0 * wgt;
-                        }
-                    }
-                }
-#pragma unroll
-                for (int j = 0; j < Q; j++) {
-                    const int P = (j << S) | c;
-                    o[P] = acc[j][0];
-                    if constexpr (KIND == KIND_NEGA) o[P + L] = acc[j][1];
-                }
-            }
-            err_update(ba.err, (int)(&pr - ba.prods), em);
+            // 2^S / BT iterations for every thread
+            for (int c = tid; c < (1 << S); c += BT) em = fmaxf(em, combine_pos_rt(ba, pr, tb, c));
+            err_update(ba.err, prod, em);
         }
     }
+}
+
+// Small items (S <= 12) as calls, which keeps the kernel's code size and
+// compile time in bounds; the large ones inlined.  The arguments are uniform
+// over the workgroup, but a call passes them in vector registers, where every
+// buffer load through a table pointer would become a loop over lanes: the
+// callee works on scalar copies of them.
+template <int KIND, int S, int E, int RB>
+__device__ __noinline__ void run_item_call(cplx<float>* lds, const BArgs& ba_in,
+                                           const BProd& pr_in, const BTables& tb_in, int prod,
+                                           int q, int tid, int* sh)
+{
+    const BArgs ba = uniform_copy(ba_in);
+    const BProd pr = uniform_copy(pr_in);
+    const BTables tb = uniform_copy(tb_in);
+    run_item<KIND, S, E, RB>(lds, ba, pr, tb, __builtin_amdgcn_readfirstlane(prod),
+                                 __builtin_amdgcn_readfirstlane(q), tid, sh);
 }
 
 template <int KIND, int S, int RB>
 __device__ __forceinline__ void dispatch_sb(cplx<float>* lds, const BArgs& ba, const BProd& pr,
-                                            const BTables& tb, int q, int tid, int* sh)
+                                            const BTables& tb, int prod, int q, int tid, int* sh)
 {
     constexpr int E = S >= 14 ? 5 : (S == 13 ? 4 : 3);  // NT <= 512
     constexpr int R = RB < E ? RB : E;
-    if constexpr (S == SMAX) {
-        switch (pr.SB) {
-            case 0: run_item<KIND, S, 0, E, R>(lds, ba, pr, tb, q, tid, sh); break;
-            case 1: run_item<KIND, S, 1, E, R>(lds, ba, pr, tb, q, tid, sh); break;
-            case 2: run_item<KIND, S, 2, E, R>(lds, ba, pr, tb, q, tid, sh); break;
-            case 3: run_item<KIND, S, 3, E, R>(lds, ba, pr, tb, q, tid, sh); break;
-            case 4: run_item<KIND, S, 4, E, R>(lds, ba, pr, tb, q, tid, sh); break;
-            default: break;
-        }
-    } else {
-        run_item<KIND, S, 0, E, R>(lds, ba, pr, tb, q, tid, sh);
-    }
+    if constexpr (S >= SMAX - 1)
+        run_item<KIND, S, E, R>(lds, ba, pr, tb, prod, q, tid, sh);
+    else
+        run_item_call<KIND, S, E, R>(lds, ba, pr, tb, prod, q, tid, sh);
 }
 
 template <int KIND, int RB>
 __device__ __forceinline__ void dispatch_s(cplx<float>* lds, const BArgs& ba, const BProd& pr,
-                                           const BTables& tb, int q, int tid, int* sh)
+                                           const BTables& tb, int prod, int q, int tid, int* sh)
 {
     switch (pr.S) {
-        case 6: dispatch_sb<KIND, 6, RB>(lds, ba, pr, tb, q, tid, sh); break;
-        case 7: dispatch_sb<KIND, 7, RB>(lds, ba, pr, tb, q, tid, sh); break;
-        case 8: dispatch_sb<KIND, 8, RB>(lds, ba, pr, tb, q, tid, sh); break;
-        case 9: dispatch_sb<KIND, 9, RB>(lds, ba, pr, tb, q, tid, sh); break;
-        case 10: dispatch_sb<KIND, 10, RB>(lds, ba, pr, tb, q, tid, sh); break;
-        case 11: dispatch_sb<KIND, 11, RB>(lds, ba, pr, tb, q, tid, sh); break;
-        case 12: dispatch_sb<KIND, 12, RB>(lds, ba, pr, tb, q, tid, sh); break;
-        case 13: dispatch_sb<KIND, 13, RB>(lds, ba, pr, tb, q, tid, sh); break;
-        case 14: dispatch_sb<KIND, 14, RB>(lds, ba, pr, tb, q, tid, sh); break;
+        case 5: dispatch_sb<KIND, 5, RB>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 6: dispatch_sb<KIND, 6, RB>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 7: dispatch_sb<KIND, 7, RB>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 8: dispatch_sb<KIND, 8, RB>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 9: dispatch_sb<KIND, 9, RB>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 10: dispatch_sb<KIND, 10, RB>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 11: dispatch_sb<KIND, 11, RB>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 12: dispatch_sb<KIND, 12, RB>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 13: dispatch_sb<KIND, 13, RB>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 14: dispatch_sb<KIND, 14, RB>(lds, ba, pr, tb, prod, q, tid, sh); break;
         default: break;
     }
 }
@@ -394,21 +653,43 @@ __global__ __launch_bounds__(BT) void batched_kernel(BArgs ba)
     __shared__ int sh_item;
     __shared__ int sh_last;
     const int tid = threadIdx.x;
-    for (;;) {
-        if (tid == 0) sh_item = atomicAdd(ba.next_item, 1);
-        __syncthreads();
-        const int it = sh_item;
-        __syncthreads();
-        if (it >= ba.n_items) break;  // every workgroup ends here
-        const int2 item = ba.items[it];
-        const BProd& pr = ba.prods[item.x];
-        const BTables tb = ba.tabs[pr.tabset];
-        if (pr.kind == KIND_NEGA)
-            dispatch_s<KIND_NEGA, RB>(lds, ba, pr, tb, item.y, tid, &sh_last);
-        else
-            dispatch_s<KIND_LINEAR, RB>(lds, ba, pr, tb, item.y, tid, &sh_last);
-        __syncthreads();
+    const int home = (int)(blockIdx.x % NQ);
+    for (int r = 0; r < NQ; r++) {
+        const int x = (home + r) % NQ;
+        for (;;) {
+            if (tid == 0) sh_item = atomicAdd(ba.next_item + x, 1);
+            __syncthreads();
+            const int it = sh_item;
+            __syncthreads();
+            if (it >= ba.qlen[x]) break;  // queue x is exhausted
+            // the item's records are read from memory, which the compiler
+            // cannot prove uniform: scalar copies keep the table resources in
+            // scalar registers (a buffer load through a resource in vector
+            // registers becomes a loop over lanes)
+            const int2 it2 = ba.items[ba.qbeg[x] + it];
+            const int2 item = make_int2(__builtin_amdgcn_readfirstlane(it2.x),
+                                        __builtin_amdgcn_readfirstlane(it2.y));
+            const BProd pr = uniform_copy(ba.prods[item.x]);
+            const BTables tb = uniform_copy(ba.tabs[pr.tabset]);
+            if (pr.kind == KIND_NEGA)
+                dispatch_s<KIND_NEGA, RB>(lds, ba, pr, tb, item.x, item.y, tid, &sh_last);
+            else
+                dispatch_s<KIND_LINEAR, RB>(lds, ba, pr, tb, item.x, item.y, tid, &sh_last);
+            __syncthreads();
+        }
     }
+}
+
+// One thread per position c < 2^SMAX of one split product (blockIdx.x: the
+// product's index in the split list).
+__global__ __launch_bounds__(CT) void batched_combine_kernel(BArgs ba)
+{
+    const int prod = __builtin_amdgcn_readfirstlane(ba.split[blockIdx.x]);
+    const BProd pr = uniform_copy(ba.prods[prod]);
+    const BTables tb = uniform_copy(ba.tabs[pr.tabset]);
+    const int c = blockIdx.y * CT + threadIdx.x;
+    const float em = combine_pos_rt(ba, pr, tb, c);
+    err_update(ba.err, prod, em);
 }
 
 }  // namespace
@@ -416,13 +697,22 @@ __global__ __launch_bounds__(BT) void batched_kernel(BArgs ba)
 struct gpufft_batched_plan_s {
     int count;
     int n_items;
+    int n_split;      // products with SB > 0
     int wa, ka, wb, kb;
     int rb;
     int grid;
+    int cmb;          // 1: combine split products in the persistent kernel
+    int qbeg[NQ], qlen[NQ];
     BProd* d_prods;
     int2* d_items;
     BTables* d_tabs;
-    int* d_counters;  // [0] work counter, [1..count] arrival counters
+    int* d_counters;  // [0, NQ) queue counters, [NQ, NQ + count) arrival counters
+    int* d_split;
+    BSched sched;
+    BSched* d_sched;
+    int stash_slots;
+    int stash_grid;   // workgroups the stash is sized for
+    cplx<float>* stash;
     cplx<float>* scratch;
     void* wc;
     void* wci;
@@ -440,7 +730,8 @@ void release(gpufft_batched_plan_s* p)
     if (!p) return;
     if (p->graph) (void)hipGraphExecDestroy(p->graph);
     for (void* t : p->tables) (void)hipFree(t);
-    void* bufs[7] = {p->d_prods, p->d_items, p->d_tabs, p->d_counters, p->scratch, p->wc, p->wci};
+    void* bufs[10] = {p->d_prods, p->d_items, p->d_tabs, p->d_counters, p->d_split, p->d_sched,
+                      p->stash, p->scratch, p->wc, p->wci};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     delete p;
@@ -457,12 +748,86 @@ template <typename V> int upload(V** dst, const std::vector<V>& src, hipStream_t
     return GPUFFT_OK;
 }
 
+// Groups of equal weight, the order of their inverse transforms and the
+// stash slots of their partial sums (a slot is reused once its group has
+// been inverted, by a group whose first contribution comes with a later a
+// limb).  Returns the number of group slots.
+int build_sched(int wa, int ka, int wb, int kb, BSched& sc)
+{
+    std::memset(&sc, 0, sizeof(sc));
+    std::map<int, int> gid;
+    std::vector<int> gfirst;  // a limb of each group's first contribution
+    for (int ia = 0; ia < ka; ia++)
+        for (int jb = 0; jb < kb; jb++) {
+            const int w = wa * ia + wb * jb;
+            auto f = gid.find(w);
+            if (f == gid.end()) {
+                f = gid.emplace(w, sc.G).first;
+                sc.gshift[sc.G] = w;
+                gfirst.push_back(ia);
+                sc.cfirst[ia * kb + jb] = 1;
+                sc.G++;
+            }
+            sc.cgroup[ia * kb + jb] = f->second;
+            sc.gdone[f->second] = ia;  // the last a limb seen so far
+        }
+    for (int ia = 0; ia < ka; ia++) {
+        // prefer a group that already has a partial sum: it is read anyway
+        sc.dgroup[ia] = -1;
+        for (int g = 0; g < sc.G; g++)
+            if (sc.gdone[g] == ia && (sc.dgroup[ia] < 0 || (gfirst[g] < ia && gfirst[sc.dgroup[ia]] == ia)))
+                sc.dgroup[ia] = g;
+        sc.dload[ia] = sc.dgroup[ia] >= 0 && gfirst[sc.dgroup[ia]] < ia ? 1 : 0;
+    }
+    // slot intervals [gfirst, gdone], groups in order of first contribution
+    std::vector<int> slot_free_after;  // gdone of the slot's last group
+    for (int g = 0; g < sc.G; g++) {
+        const bool direct = sc.dgroup[sc.gdone[g]] == g;
+        sc.gslot[g] = -1;
+        if (direct && gfirst[g] == sc.gdone[g]) continue;  // summed in registers only
+        int s = -1;
+        for (int k = 0; k < (int)slot_free_after.size() && s < 0; k++)
+            if (slot_free_after[k] < gfirst[g]) s = k;
+        if (s < 0) {
+            s = (int)slot_free_after.size();
+            slot_free_after.push_back(0);
+        }
+        slot_free_after[s] = sc.gdone[g];
+        sc.gslot[g] = s;
+    }
+    // a group's contributions before its completing limb go to its slot; the
+    // first of them is a store
+    for (int k = 0; k < ka * kb; k++) sc.cfirst[k] = 0;
+    std::vector<int> seen(sc.G, 0);
+    for (int ia = 0; ia < ka; ia++)
+        for (int jb = 0; jb < kb; jb++) {
+            const int g = sc.cgroup[ia * kb + jb];
+            const bool to_regs = sc.dgroup[ia] == g;
+            if (!to_regs && !seen[g]) sc.cfirst[ia * kb + jb] = 1;
+            if (!to_regs) seen[g] = 1;
+        }
+    return (int)slot_free_after.size();
+}
+
+int alloc_stash(gpufft_batched_plan_s* p, int grid)
+{
+    if (p->stash_slots == 0 || grid <= p->stash_grid) return GPUFFT_OK;
+    if (p->stash) (void)hipFree(p->stash);
+    p->stash = nullptr;
+    p->stash_grid = 0;
+    hipError_t e = hipMalloc((void**)&p->stash,
+                             sizeof(cplx<float>) * ((size_t)grid * p->stash_slots << SMAX));
+    if (e != hipSuccess) return fail_hip(e, "gpufft_batched_plan: stash");
+    p->stash_grid = grid;
+    return GPUFFT_OK;
+}
+
 int launch(gpufft_batched_plan_s* p, const int* a, const int* b, long long* out, double* err,
            hipStream_t st)
 {
     const char* who = "gpufft_batched_execute";
-    hipLaunchKernelGGL(zero_ints, dim3((p->count + 1 + 255) / 256), dim3(256), 0, st, p->d_counters,
-                       p->count + 1);
+    hipLaunchKernelGGL(zero_ints, dim3((p->count + NQ + 255) / 256), dim3(256), 0, st,
+                       p->d_counters, p->count + NQ);
     GFA_CHECK_LAUNCH(who);
     if (err) {
         hipLaunchKernelGGL(zero_doubles, dim3((p->count + 255) / 256), dim3(256), 0, st, err, p->count);
@@ -472,13 +837,21 @@ int launch(gpufft_batched_plan_s* p, const int* a, const int* b, long long* out,
     ba.prods = p->d_prods;
     ba.items = p->d_items;
     ba.tabs = p->d_tabs;
-    ba.n_items = p->n_items;
+    for (int x = 0; x < NQ; x++) {
+        ba.qbeg[x] = p->qbeg[x];
+        ba.qlen[x] = p->qlen[x];
+    }
     ba.next_item = p->d_counters;
-    ba.arrive = p->d_counters + 1;
+    ba.arrive = p->d_counters + NQ;
+    ba.split = p->d_split;
+    ba.cmb = p->cmb;
     ba.a = a;
     ba.b = b;
     ba.out = out;
     ba.scratch = p->scratch;
+    ba.sched = p->d_sched;
+    ba.stash = p->stash;
+    ba.stash_slots = p->stash_slots;
     ba.wc = (const cplx<float>*)p->wc;
     ba.wci = (const cplx<float>*)p->wci;
     ba.wa = p->wa;
@@ -499,6 +872,11 @@ int launch(gpufft_batched_plan_s* p, const int* a, const int* b, long long* out,
         default: hipLaunchKernelGGL(batched_kernel<5>, dim3(p->grid), dim3(BT), 0, st, ba); break;
     }
     GFA_CHECK_LAUNCH(who);
+    if (!p->cmb && p->n_split > 0) {
+        hipLaunchKernelGGL(batched_combine_kernel, dim3(p->n_split, (1 << SMAX) / CT), dim3(CT), 0,
+                           st, ba);
+        GFA_CHECK_LAUNCH(who);
+    }
     return GPUFFT_OK;
 }
 
@@ -514,9 +892,10 @@ int gpufft_batched_plan_create(gpufft_batched_plan* out, const gpufft_product_de
     if (!out || !descs) return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": NULL argument");
     *out = nullptr;
     if (count < 1) return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": count < 1");
-    if (a_limbs < 1 || b_limbs < 1 || a_limbs * b_limbs > 64 ||
+    if (a_limbs < 1 || b_limbs < 1 || a_limbs * b_limbs > MAXP ||
         (a_limbs > 1 && (a_limb_bits < 1 || a_limb_bits > 16)) ||
-        (b_limbs > 1 && (b_limb_bits < 1 || b_limb_bits > 16)))
+        (b_limbs > 1 && (b_limb_bits < 1 || b_limb_bits > 16)) ||
+        (a_limbs - 1) * a_limb_bits + (b_limbs - 1) * b_limb_bits > 62)
         return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": invalid limb split");
     hipStream_t st = (hipStream_t)stream;
     gpufft_batched_plan_s* p = new gpufft_batched_plan_s();
@@ -526,16 +905,22 @@ int gpufft_batched_plan_create(gpufft_batched_plan* out, const gpufft_product_de
     p->wb = b_limbs > 1 ? b_limb_bits : 0;
     p->kb = b_limbs;
     p->rb = 5;
+    p->cmb = 0;
     int dev = 0, cus = 256;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     // persistent grid: one 128 KiB-LDS workgroup per CU is resident at a time
     p->grid = cus;
 
-    std::map<std::pair<int, int>, int> tabset;
+    std::map<int, int> tabset;  // n -> table set (both kinds use the FFNT tables of L = 2^n)
     std::vector<BTables> tabs;
     long long scratch = 0;
-    const int nprod = p->ka * p->kb;
+    // group slots after the kb - 1 stashed b-limb spectra
+    const int gslots = build_sched(p->wa, p->ka, p->wb, p->kb, p->sched);
+    for (int g = 0; g < p->sched.G; g++)
+        if (p->sched.gslot[g] >= 0) p->sched.gslot[g] += p->kb - 1;
+    p->stash_slots = p->kb - 1 + gslots;
+    const int ngrp = p->sched.G;
     for (int i = 0; i < count; i++) {
         const gpufft_product_desc& d = descs[i];
         BProd pr{};
@@ -544,9 +929,9 @@ int gpufft_batched_plan_create(gpufft_batched_plan* out, const gpufft_product_de
             return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": unknown product kind");
         }
         pr.kind = d.kind == GPUFFT_PRODUCT_NEGACYCLIC ? KIND_NEGA : KIND_LINEAR;
-        pr.N = 1 << d.ring_power;
-        pr.n = pr.kind == KIND_NEGA ? d.ring_power - 1 : d.ring_power + 1;
-        if (d.ring_power < 2 || pr.n < SMIN || pr.n > SMAX + SBMAX) {
+        // NEGACYCLIC: FFNT of ring N (L = N/2); LINEAR: FFNT of ring 2N (L = N)
+        pr.n = pr.kind == KIND_NEGA ? d.ring_power - 1 : d.ring_power;
+        if (pr.n < SMIN || pr.n > SMAX + SBMAX) {
             release(p);
             return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": ring_power out of range");
         }
@@ -558,14 +943,13 @@ int gpufft_batched_plan_create(gpufft_batched_plan* out, const gpufft_product_de
         pr.SB = pr.n - pr.S;
         pr.a_off = d.a_offset;
         pr.b_off = d.b_offset;
+        pr.N = 1 << d.ring_power;
         pr.out_off = d.out_offset;
-        auto key = std::make_pair(pr.kind, pr.n);
-        auto f = tabset.find(key);
+        auto f = tabset.find(pr.n);
         if (f == tabset.end()) {
             BTables t{};
             void* bufs[4] = {nullptr, nullptr, nullptr, nullptr};
-            const int nb = pr.kind == KIND_NEGA ? 4 : 2;
-            for (int k = 0; k < nb; k++) {
+            for (int k = 0; k < 4; k++) {
                 hipError_t e = hipMalloc(&bufs[k], sizeof(cplx<float>) << pr.n);
                 if (e != hipSuccess) {
                     release(p);
@@ -574,16 +958,10 @@ int gpufft_batched_plan_create(gpufft_batched_plan* out, const gpufft_product_de
                 p->tables.push_back(bufs[k]);
             }
             int s = 0;
-            if (pr.kind == KIND_NEGA) {
-                s |= gpufft_generate_table_f32(bufs[0], GPUFFT_TABLE_FFNT_ROOT, pr.n, 0, st);
-                s |= gpufft_generate_table_f32(bufs[1], GPUFFT_TABLE_FFNT_ROOT, pr.n, 1, st);
-                s |= gpufft_generate_table_f32(bufs[2], GPUFFT_TABLE_FFNT_TWIST, pr.n, 0, st);
-                s |= gpufft_generate_table_f32(bufs[3], GPUFFT_TABLE_FFNT_TWIST, pr.n, 1, st);
-            } else {
-                // cyclic length L = 2^n: FFT<T>(L/2) tables (L/2 entries)
-                s |= gpufft_generate_table_f32(bufs[0], GPUFFT_TABLE_FFT, pr.n - 1, 0, st);
-                s |= gpufft_generate_table_f32(bufs[1], GPUFFT_TABLE_FFT, pr.n - 1, 1, st);
-            }
+            s |= gpufft_generate_table_f32(bufs[0], GPUFFT_TABLE_FFNT_ROOT, pr.n, 0, st);
+            s |= gpufft_generate_table_f32(bufs[1], GPUFFT_TABLE_FFNT_ROOT, pr.n, 1, st);
+            s |= gpufft_generate_table_f32(bufs[2], GPUFFT_TABLE_FFNT_TWIST, pr.n, 0, st);
+            s |= gpufft_generate_table_f32(bufs[3], GPUFFT_TABLE_FFNT_TWIST, pr.n, 1, st);
             if (s) {
                 release(p);
                 return s;
@@ -592,32 +970,52 @@ int gpufft_batched_plan_create(gpufft_batched_plan* out, const gpufft_product_de
             t.iroot = (const cplx<float>*)bufs[1];
             t.twist = (const cplx<float>*)bufs[2];
             t.untwist = (const cplx<float>*)bufs[3];
-            f = tabset.emplace(key, (int)tabs.size()).first;
+            f = tabset.emplace(pr.n, (int)tabs.size()).first;
             tabs.push_back(t);
         }
         pr.tabset = f->second;
         if (pr.SB > 0) {
             pr.scratch_off = scratch;
-            scratch += (long long)nprod << pr.n;
+            scratch += (long long)ngrp << pr.n;
         }
         p->prods.push_back(pr);
     }
-    // items, largest products first (cost ~ L log L per sub-problem)
+    // products largest first (cost ~ L log L), each dealt to the least loaded
+    // queue; a queue lists its products' sub-problems in that order
+    auto cost = [&](int i) { return (long long)p->prods[i].n << p->prods[i].n; };
     std::vector<int> order(count);
     for (int i = 0; i < count; i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int x, int y) { return p->prods[x].S > p->prods[y].S; });
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost(x) > cost(y); });
+    std::vector<int> queue_of(count);
+    long long load[NQ] = {};
+    for (int i : order) {
+        const int x = (int)(std::min_element(load, load + NQ) - load);
+        queue_of[i] = x;
+        load[x] += cost(i);
+    }
     std::vector<int2> items;
+    std::vector<int> split;
+    for (int x = 0; x < NQ; x++) {
+        p->qbeg[x] = (int)items.size();
+        for (int i : order)
+            if (queue_of[i] == x)
+                for (int q = 0; q < (1 << p->prods[i].SB); q++) items.push_back(make_int2(i, q));
+        p->qlen[x] = (int)items.size() - p->qbeg[x];
+    }
     for (int i : order)
-        for (int q = 0; q < (1 << p->prods[i].SB); q++) items.push_back(make_int2(i, q));
+        if (p->prods[i].SB > 0) split.push_back(i);
     p->n_items = (int)items.size();
+    p->n_split = (int)split.size();
+    const std::vector<BSched> schedv(1, p->sched);
     int s;
     if ((s = upload(&p->d_prods, p->prods, st)) || (s = upload(&p->d_items, items, st)) ||
-        (s = upload(&p->d_tabs, tabs, st))) {
+        (s = upload(&p->d_tabs, tabs, st)) || (s = upload(&p->d_split, split, st)) ||
+        (s = upload(&p->d_sched, schedv, st)) ||
+        (s = alloc_stash(p, p->grid))) {
         release(p);
         return s;
     }
-    hipError_t e = hipMalloc((void**)&p->d_counters, sizeof(int) * (count + 1));
+    hipError_t e = hipMalloc((void**)&p->d_counters, sizeof(int) * (count + NQ));
     if (e == hipSuccess && scratch > 0) e = hipMalloc((void**)&p->scratch, sizeof(cplx<float>) * scratch);
     if (e == hipSuccess) e = hipMalloc(&p->wc, sizeof(cplx<float>) * 32);
     if (e == hipSuccess) e = hipMalloc(&p->wci, sizeof(cplx<float>) * 32);
@@ -651,12 +1049,35 @@ int gpufft_batched_plan_set_tuning(gpufft_batched_plan plan, int radix_bits, int
     if (!plan) return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": plan is NULL");
     if (radix_bits < 3 || radix_bits > 5 || grid < 1)
         return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": invalid tuning");
+    if (int s = alloc_stash(plan, grid)) return s;
     plan->rb = radix_bits;
     plan->grid = grid;
     if (plan->graph) {
         (void)hipGraphExecDestroy(plan->graph);
         plan->graph = nullptr;
     }
+    return ok();
+}
+
+int gpufft_batched_plan_set_combine(gpufft_batched_plan plan, int in_kernel)
+{
+    const char* who = "gpufft_batched_plan_set_combine";
+    if (!plan) return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": plan is NULL");
+    if (in_kernel != 0 && in_kernel != 1)
+        return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": in_kernel must be 0 or 1");
+    plan->cmb = in_kernel;
+    if (plan->graph) {
+        (void)hipGraphExecDestroy(plan->graph);
+        plan->graph = nullptr;
+    }
+    return ok();
+}
+
+int gpufft_batched_plan_get_combine(gpufft_batched_plan plan, int* in_kernel)
+{
+    const char* who = "gpufft_batched_plan_get_combine";
+    if (!plan || !in_kernel) return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": NULL argument");
+    *in_kernel = plan->cmb;
     return ok();
 }
 
@@ -667,7 +1088,7 @@ int gpufft_batched_plan_info(gpufft_batched_plan plan, int* n_items, long long* 
     if (n_items) *n_items = plan->n_items;
     long long s = 0;
     for (const BProd& pr : plan->prods)
-        if (pr.SB > 0) s += (long long)plan->ka * plan->kb << pr.n;
+        if (pr.SB > 0) s += (long long)plan->sched.G << pr.n;
     if (scratch_elems) *scratch_elems = s;
     if (radix_bits) *radix_bits = plan->rb;
     if (grid) *grid = plan->grid;

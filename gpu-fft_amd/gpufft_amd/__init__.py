@@ -84,6 +84,8 @@ _SIGS = {
     "gpufft_batched_plan_create": [_P, _P, _I, _I, _I, _I, _I, _P],
     "gpufft_batched_plan_destroy": [_P],
     "gpufft_batched_plan_set_tuning": [_P, _I, _I],
+    "gpufft_batched_plan_set_combine": [_P, _I],
+    "gpufft_batched_plan_get_combine": [_P, _P],
     "gpufft_batched_plan_info": [_P, _P, _P, _P, _P],
     "gpufft_batched_execute": [_P, _P, _P, _P, _P, _P],
     "gpufft_batched_graph_launch": [_P, _P, _P, _P, _P, _P],
@@ -351,6 +353,18 @@ class BatchedPlan:
 
     def set_tuning(self, radix_bits, grid):
         check(_lib.gpufft_batched_plan_set_tuning(self.handle, radix_bits, grid))
+
+    @property
+    def combine_in_kernel(self):
+        """True: split products are combined inside the persistent kernel
+        (one launch); False: by a second, streaming kernel (default)."""
+        v = ctypes.c_int()
+        check(_lib.gpufft_batched_plan_get_combine(self.handle, ctypes.byref(v)))
+        return bool(v.value)
+
+    @combine_in_kernel.setter
+    def combine_in_kernel(self, on):
+        check(_lib.gpufft_batched_plan_set_combine(self.handle, 1 if on else 0))
 
     def info(self):
         n, s, rb, g = ctypes.c_int(), ctypes.c_longlong(), ctypes.c_int(), ctypes.c_int()

@@ -116,9 +116,14 @@ def tune_polymul(plan, run, stream, key=None, reps=5, use_db=True, db_file=None)
 
 
 def tune_batched(plan, run, stream, key, grids=None, reps=3, use_db=True, db_file=None):
+    """Radix of the LDS steps, persistent grid size and where split products
+    are combined (0: separate kernel, 1: inside the persistent kernel)."""
     import torch
     cus = torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
     grids = grids or (cus // 2, cus)
-    cands = [(rb, g) for rb in (3, 4, 5) for g in grids]
-    return tune(key, cands, lambda c: plan.set_tuning(c[0], c[1]), run, stream, reps, use_db,
-                db_file)
+    cands = [(rb, g, cmb) for cmb in (0, 1) for rb in (3, 4, 5) for g in grids]
+
+    def apply(c):
+        plan.set_tuning(c[0], c[1])
+        plan.combine_in_kernel = bool(c[2]) if len(c) > 2 else False
+    return tune(key, cands, apply, run, stream, reps, use_db, db_file)

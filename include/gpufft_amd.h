@@ -238,13 +238,19 @@ int gpufft_ckks_decode_f64(const int64_t* coeffs, void* slots, const void* table
  *   GPUFFT_PRODUCT_NEGACYCLIC  out[0..N)  = a * b mod X^N + 1
  *   GPUFFT_PRODUCT_LINEAR      out[0..2N) = a * b (2N-1 coefficients, then 0)
  * with N = 2^ring_power coefficients per operand (NEGACYCLIC: ring_power
- * 7..19, LINEAR: 5..17) at the given element offsets of the int32 inputs a,
- * b and the int64 output.  Arithmetic is FP32 with the operands split into
- * balanced limbs (as gpufft_polymul_i32) and exact int64 recombination.
- * The plan (created once; it allocates its tables, work list and scratch)
- * runs every product in one persistent kernel: gpufft_batched_execute
- * enqueues it on `stream`; gpufft_batched_graph_launch captures it into a
- * HIP graph on first use (per buffer set) and replays the graph.  err_stats
+ * 6..19, LINEAR: 5..18) at the given element offsets of the int32 inputs a,
+ * b and the int64 output.  A LINEAR product is computed as the negacyclic
+ * product of ring 2N of the zero-padded operands.  Arithmetic is FP32 with
+ * the operands split into balanced limbs (as gpufft_polymul_i32) and exact
+ * int64 recombination.  The plan (created once; it allocates its tables,
+ * work list and scratch) runs every product in one persistent kernel, whose
+ * workgroups take sub-problems from 8 per-XCD work queues.  The top inverse
+ * stages of products split over several workgroups run in a second, streaming
+ * kernel (default) or in the persistent kernel itself, by the last workgroup
+ * of the product to finish (gpufft_batched_plan_set_combine(plan, 1)).
+ * gpufft_batched_execute enqueues the launches on `stream`;
+ * gpufft_batched_graph_launch captures them into a HIP graph on first use
+ * (per buffer set) and replays the graph.  err_stats
  * (optional, double [count]) receives each product's largest distance of a
  * limb output to its rounding (< 0.5: every limb was rounded exactly).
  * Calls on one plan must not overlap. */
@@ -263,6 +269,10 @@ int gpufft_batched_plan_destroy(gpufft_batched_plan plan);
 /* radix of the LDS steps (2^3..2^5) and persistent grid size (workgroups);
  * chosen by the autotuner (gpufft_amd.autotune) */
 int gpufft_batched_plan_set_tuning(gpufft_batched_plan plan, int radix_bits, int grid);
+/* where split products are combined: 0 = separate streaming kernel (default),
+ * 1 = in the persistent kernel (one launch); chosen by the autotuner */
+int gpufft_batched_plan_set_combine(gpufft_batched_plan plan, int in_kernel);
+int gpufft_batched_plan_get_combine(gpufft_batched_plan plan, int* in_kernel);
 int gpufft_batched_plan_info(gpufft_batched_plan plan, int* n_items, long long* scratch_elems,
                              int* radix_bits, int* grid);
 int gpufft_batched_execute(gpufft_batched_plan plan, const int32_t* a, const int32_t* b,

@@ -77,6 +77,8 @@ def test_polymul_plan_validation_on_host():
     assert G.lib().gpufft_polymul_plan_get_combine(None) == -1
     assert G.lib().gpufft_polymul_plan_set_a_center(None, 19) == G.ERR_NULL_POINTER
     assert G.lib().gpufft_polymul_plan_get_a_center(None) == -1
+    assert G.lib().gpufft_batched_plan_set_combine(None, 1) == G.ERR_NULL_POINTER
+    assert G.lib().gpufft_batched_plan_get_combine(None, None) == G.ERR_NULL_POINTER
 
 
 def test_table_generation_validation_on_host():
@@ -85,3 +87,14 @@ def test_table_generation_validation_on_host():
         G.check(G.lib().gpufft_generate_table_f64(0x1000, 9, 4, 0, None))
     with pytest.raises(ValueError):
         G.check(G.lib().gpufft_generate_table_f64(None, 0, 4, 0, None))
+
+
+def test_batched_plan_validation_on_host():
+    """Ring powers outside NEGACYCLIC 6..19 / LINEAR 5..18 and unknown kinds
+    are refused before any device allocation."""
+    import gpufft_amd as G
+    for kind, rp in ((G.PRODUCT_NEGACYCLIC, 5), (G.PRODUCT_NEGACYCLIC, 20),
+                     (G.PRODUCT_LINEAR, 4), (G.PRODUCT_LINEAR, 19), (7, 12)):
+        descs, _, _ = G.pack_products([kind], [rp])
+        with pytest.raises(ValueError):
+            G.BatchedPlan(descs, stream=0)

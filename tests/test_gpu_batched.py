@@ -80,16 +80,39 @@ def test_batched_graph_replays():
     assert np.array_equal(to_host(out), exact)
 
 
+@pytest.mark.parametrize("in_kernel", [False, True])
 @pytest.mark.parametrize("rb", [3, 4, 5])
 @pytest.mark.parametrize("grid", [16, 256])
-def test_batched_tuning_candidates_agree(rb, grid):
+def test_batched_tuning_candidates_agree(rb, grid, in_kernel):
     import torch
 
     import gpufft_amd as G
-    kinds, rps = [1, 0, 0, 1], [16, 17, 12, 11]
+    kinds, rps = [1, 0, 0, 1, 1], [16, 17, 12, 11, 18]
     descs, a, b, exact = _workload(kinds, rps, 0x6770756666740084)
     plan = G.BatchedPlan(descs, **LIMBS)
     plan.set_tuning(rb, grid)
+    plan.combine_in_kernel = in_kernel
+    assert plan.combine_in_kernel == in_kernel
     out = torch.zeros(len(exact), dtype=torch.int64, device="cuda")
     plan.execute(to_dev(a.astype(np.int32)), to_dev(b.astype(np.int32)), out)
     assert np.array_equal(to_host(out), exact)
+
+
+def test_batched_range_edges():
+    """Smallest and largest ring powers of both kinds (NEGACYCLIC 6 and 19,
+    LINEAR 5 and 18) in one plan, both combine placements."""
+    import torch
+
+    import gpufft_amd as G
+    kinds, rps = [0, 1, 0, 1], [6, 5, 19, 18]
+    descs, a, b, exact = _workload(kinds, rps, 0x67707566667400c4)
+    plan = G.BatchedPlan(descs, **LIMBS)
+    da, db = to_dev(a.astype(np.int32)), to_dev(b.astype(np.int32))
+    for in_kernel in (False, True):
+        plan.combine_in_kernel = in_kernel
+        out = torch.full((len(exact),), -1, dtype=torch.int64, device="cuda")
+        err = torch.zeros(len(descs), dtype=torch.float64, device="cuda")
+        plan.execute(da, db, out, err)
+        e = to_host(err)
+        assert np.all(e < 0.5), e
+        assert np.array_equal(to_host(out), exact), in_kernel
