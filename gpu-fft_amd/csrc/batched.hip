@@ -162,52 +162,76 @@ template <typename T> __device__ __forceinline__ gptr<T> to_global(T* p)
     return (gptr<T>)p;
 }
 
+// Balanced base-2^w digit i of an int32 v in 32-bit arithmetic (as
+// limb_digit): with o = sum_{j < k-1} 2^(w-1) 2^(w j) < 2^s, s = w (k-1) <= 30,
+// the digits below the top are bits of v + o mod 2^32, and the top digit
+// floor((v + o) / 2^s) = (v >> s) + (((v mod 2^s) + o) >> s), whose second
+// sum stays below 2^31.
+__device__ __forceinline__ int limb_digit32(int v, int w, int i, int k, int o)
+{
+    if (k == 1) return v;
+    const int s = w * (k - 1);
+    if (i == k - 1) return (v >> s) + (((v & ((1 << s) - 1)) + o) >> s);
+    const unsigned u = (unsigned)v + (unsigned)o;
+    return (int)((u >> (w * i)) & ((1u << w) - 1)) - (1 << (w - 1));
+}
+
 // Sub-problem q of one limb of one operand into LDS: for each of the
 // workgroup's 2^S positions c, row q of the top SB forward stages over the
 // folded inputs at j 2^S + c (Q = 2^SB partners), times the twist
 // (coefficients C as in top_row_coeffs).  LINEAR products have no upper fold
-// half.  U positions per round, their loads issued together.
+// half.  A lane takes 4 consecutive positions with one 16-byte load per
+// partner (operand offsets are multiples of 4 elements), U such groups per
+// round with all their loads issued together.
 template <int KIND, int S, int Q, int E>
 __device__ __forceinline__ void item_load(cplx<float>* lds, gptr<const cplx<float>> twist,
                                           gptr<const int> x, int L, int w, int limb, int k,
-                                          long long o, const cplx<float>* C, int tid)
+                                          int o, const cplx<float>* C, int tid)
 {
+    typedef int i32x4 __attribute__((ext_vector_type(4)));
+    typedef float f32x4 __attribute__((ext_vector_type(4)));
     constexpr int NT = (1 << S) >> E;
-    constexpr int EPT = 1 << E;
-    constexpr int U0 = Q >= 4 ? 2 : (Q == 2 ? 4 : 8);
-    constexpr int U = U0 < EPT ? U0 : EPT;
+    constexpr int NV = (1 << E) / 4;  // groups of 4 positions per lane
+    constexpr int U0 = Q >= 4 ? 1 : (Q == 2 ? 2 : 4);
+    constexpr int U = U0 < NV ? U0 : NV;
     if (tid >= NT) return;
-    for (int i0 = 0; i0 < EPT; i0 += U) {
-        int xr[U][Q], xi[U][Q];
-        cplx<float> tw[U];
+    for (int i0 = 0; i0 < NV; i0 += U) {
+        i32x4 xr[U][Q], xi[U][Q];
+        f32x4 tw[U][2];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const int c = tid + (i0 + u) * NT;
-            tw[u] = twist[c];
+            const int c0 = (tid + (i0 + u) * NT) * 4;
+            tw[u][0] = *(gptr<const f32x4>)(twist + c0);
+            tw[u][1] = *(gptr<const f32x4>)(twist + c0 + 2);
 #pragma unroll
             for (int j = 0; j < Q; j++) {
-                const int P = (j << S) | c;
-                xr[u][j] = x[P];
-                if constexpr (KIND == KIND_NEGA) xi[u][j] = x[P + L];
+                const int P = (j << S) | c0;
+                xr[u][j] = *(gptr<const i32x4>)(x + P);
+                if constexpr (KIND == KIND_NEGA) xi[u][j] = *(gptr<const i32x4>)(x + P + L);
             }
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const int c = tid + (i0 + u) * NT;
-            float ax = 0.f, ay = 0.f;
+            const int c0 = (tid + (i0 + u) * NT) * 4;
 #pragma unroll
-            for (int j = 0; j < Q; j++) {
-                const float re = (float)limb_digit(xr[u][j], w, limb, k, o);
-                if constexpr (KIND == KIND_NEGA) {
-                    const float im = (float)limb_digit(xi[u][j], w, limb, k, o);
-                    ax = ax + C[j].x * re - C[j].y * im;
-                    ay = ay + C[j].x * im + C[j].y * re;
-                } else {
-                    ax = ax + C[j].x * re;
-                    ay = ay + C[j].y * re;
+            for (int v = 0; v < 4; v++) {
+                float ax = 0.f, ay = 0.f;
+#pragma unroll
+                for (int j = 0; j < Q; j++) {
+                    const float re = (float)limb_digit32(xr[u][j][v], w, limb, k, o);
+                    if constexpr (KIND == KIND_NEGA) {
+                        const float im = (float)limb_digit32(xi[u][j][v], w, limb, k, o);
+                        ax = ax + C[j].x * re - C[j].y * im;
+                        ay = ay + C[j].x * im + C[j].y * re;
+                    } else {
+                        ax = ax + C[j].x * re;
+                        ay = ay + C[j].y * re;
+                    }
                 }
+                const f32x4 t2 = tw[u][v >> 1];
+                const cplx<float> t = (v & 1) ? mk<float>(t2[2], t2[3]) : mk<float>(t2[0], t2[1]);
+                lds[swz(c0 + v)] = t * mk<float>(ax, ay);
             }
-            lds[swz(c)] = tw[u] * mk<float>(ax, ay);
         }
     }
 }
@@ -392,7 +416,7 @@ __device__ __forceinline__ void run_item(cplx<float>* lds, const BArgs& ba, cons
     bool first_out = true;
 
     // gather with a compile-time partner count
-    auto gather = [&](gptr<const int> x, int w, int limb, int k, long long o, int t) {
+    auto gather = [&](gptr<const int> x, int w, int limb, int k, int o, int t) {
         if constexpr (QMAX == 1) {
             item_load<KIND, S, 1, E>(lds, twist, x, L, w, limb, k, o, C, t);
         } else {
@@ -449,7 +473,7 @@ __device__ __forceinline__ void run_item(cplx<float>* lds, const BArgs& ba, cons
         int t = tid;
         asm volatile("" : "+v"(t));
         __syncthreads();
-        gather(xb, ba.wb, jb, ba.kb, ba.ob, t);
+        gather(xb, ba.wb, jb, ba.kb, (int)ba.ob, t);
         run_steps_ct_active<float, MODE_STD_FWD, E, NT, S, RB, 0, NST - 1>(lds, pf, g, t, ba.wc, active);
         if (active) {
 #pragma unroll
@@ -476,7 +500,7 @@ __device__ __forceinline__ void run_item(cplx<float>* lds, const BArgs& ba, cons
         int u = tid;
         asm volatile("" : "+v"(u));
         __syncthreads();
-        gather(xa, ba.wa, ia, ba.ka, ba.oa, u);
+        gather(xa, ba.wa, ia, ba.ka, (int)ba.oa, u);
         run_steps_ct_active<float, MODE_STD_FWD, E, NT, S, RB, 0, NST - 1>(lds, pf, g, u, ba.wc,
                                                                        active);
         const int dg = __builtin_amdgcn_readfirstlane(sc.dgroup[ia]);
@@ -895,7 +919,8 @@ int gpufft_batched_plan_create(gpufft_batched_plan* out, const gpufft_product_de
     if (a_limbs < 1 || b_limbs < 1 || a_limbs * b_limbs > MAXP ||
         (a_limbs > 1 && (a_limb_bits < 1 || a_limb_bits > 16)) ||
         (b_limbs > 1 && (b_limb_bits < 1 || b_limb_bits > 16)) ||
-        (a_limbs - 1) * a_limb_bits + (b_limbs - 1) * b_limb_bits > 62)
+        (a_limbs > 1 && (a_limbs - 1) * a_limb_bits > 30) ||
+        (b_limbs > 1 && (b_limbs - 1) * b_limb_bits > 30))
         return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": invalid limb split");
     hipStream_t st = (hipStream_t)stream;
     gpufft_batched_plan_s* p = new gpufft_batched_plan_s();
@@ -938,6 +963,11 @@ int gpufft_batched_plan_create(gpufft_batched_plan* out, const gpufft_product_de
         if (d.a_offset < 0 || d.b_offset < 0 || d.out_offset < 0) {
             release(p);
             return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": negative offset");
+        }
+        if ((d.a_offset | d.b_offset) & 3) {  // 16-byte operand loads
+            release(p);
+            return fail(GPUFFT_ERR_INVALID_ARGUMENT,
+                        std::string(who) + ": a_offset and b_offset must be multiples of 4");
         }
         pr.S = std::min(pr.n, SMAX);
         pr.SB = pr.n - pr.S;

@@ -239,10 +239,13 @@ int gpufft_ckks_decode_f64(const int64_t* coeffs, void* slots, const void* table
  *   GPUFFT_PRODUCT_LINEAR      out[0..2N) = a * b (2N-1 coefficients, then 0)
  * with N = 2^ring_power coefficients per operand (NEGACYCLIC: ring_power
  * 6..19, LINEAR: 5..18) at the given element offsets of the int32 inputs a,
- * b and the int64 output.  A LINEAR product is computed as the negacyclic
+ * b (multiples of 4: operands are read 16 bytes at a time) and the int64
+ * output.  A LINEAR product is computed as the negacyclic
  * product of ring 2N of the zero-padded operands.  Arithmetic is FP32 with
- * the operands split into balanced limbs (as gpufft_polymul_i32) and exact
- * int64 recombination.  The plan (created once; it allocates its tables,
+ * the operands split into balanced limbs (as gpufft_polymul_i32; the top limb
+ * must start at bit 30 or below: (limbs - 1) * limb_bits <= 30) and exact
+ * int64 recombination.  Limb products of equal weight are summed before their
+ * inverse transform.  The plan (created once; it allocates its tables,
  * work list and scratch) runs every product in one persistent kernel, whose
  * workgroups take sub-problems from 8 per-XCD work queues.  The top inverse
  * stages of products split over several workgroups run in a second, streaming

@@ -98,3 +98,10 @@ def test_batched_plan_validation_on_host():
         descs, _, _ = G.pack_products([kind], [rp])
         with pytest.raises(ValueError):
             G.BatchedPlan(descs, stream=0)
+    descs, _, _ = G.pack_products([G.PRODUCT_NEGACYCLIC], [12])
+    for split in ((16, 3, 5, 2), (5, 2, 11, 4), (5, 0, 5, 2), (17, 2, 5, 2)):
+        with pytest.raises(ValueError, match="limb split"):
+            G.BatchedPlan(descs, *split, stream=0)
+    descs[0].b_offset = 2
+    with pytest.raises(ValueError, match="multiples of 4"):
+        G.BatchedPlan(descs, stream=0)

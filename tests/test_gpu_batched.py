@@ -116,3 +116,26 @@ def test_batched_range_edges():
         e = to_host(err)
         assert np.all(e < 0.5), e
         assert np.array_equal(to_host(out), exact), in_kernel
+
+
+@pytest.mark.parametrize("split", [(4, 3, 4, 3), (4, 3, 5, 2), (5, 2, 4, 3), (9, 1, 5, 2),
+                                   (5, 2, 9, 1), (3, 4, 3, 4)])
+def test_batched_limb_schedules(split):
+    """Limb splits other than 2 x 2: equal weights summed before one inverse
+    (3 x 3 limbs: groups of 1, 2, 3, 2, 1 products, partial sums in stash
+    slots), unequal weights inverted one by one, an unsplit operand."""
+    import torch
+
+    import gpufft_amd as G
+    kinds, rps = [0, 1, 0, 1, 0], [15, 14, 12, 9, 7]
+    descs, a, b, exact = _workload(kinds, rps, 0x6770756666740104)
+    plan = G.BatchedPlan(descs, *split)
+    da, db = to_dev(a.astype(np.int32)), to_dev(b.astype(np.int32))
+    for in_kernel in (False, True):
+        plan.combine_in_kernel = in_kernel
+        out = torch.full((len(exact),), -1, dtype=torch.int64, device="cuda")
+        err = torch.zeros(len(descs), dtype=torch.float64, device="cuda")
+        plan.execute(da, db, out, err)
+        e = to_host(err)
+        assert np.all(e < 0.5), (split, e)
+        assert np.array_equal(to_host(out), exact), (split, in_kernel)
