@@ -185,9 +185,15 @@ __device__ __forceinline__ int limb_digit32(int v, int w, int i, int k, int o)
 // round with all their loads issued together.
 template <int KIND, int S, int Q, int E>
 __device__ __forceinline__ void item_load(cplx<float>* lds, gptr<const cplx<float>> twist,
-                                          gptr<const int> x, int L, int w, int limb, int k,
-                                          int o, const cplx<float>* C, int tid)
+                                          gptr<const int> x, int L, int w_in, int limb_in,
+                                          int k_in, int o_in, const cplx<float>* C, int tid)
 {
+    // the split parameters are uniform: scalar copies keep the digit
+    // extraction free of per-lane branches
+    const int w = __builtin_amdgcn_readfirstlane(w_in);
+    const int limb = __builtin_amdgcn_readfirstlane(limb_in);
+    const int k = __builtin_amdgcn_readfirstlane(k_in);
+    const int o = __builtin_amdgcn_readfirstlane(o_in);
     typedef int i32x4 __attribute__((ext_vector_type(4)));
     typedef float f32x4 __attribute__((ext_vector_type(4)));
     constexpr int NT = (1 << S) >> E;
