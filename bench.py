@@ -215,9 +215,10 @@ def surface(args):
 def batched_uneven(args):
     """Batched uneven form: `count` products, ring sizes 2^e with e uniform in
     {10..17}, even index linear (C_C, zero-padded cyclic), odd negacyclic,
-    coefficients uniform in [-2^8, 2^8), seeded.  One persistent launch
-    replayed as a HIP graph (FP32 split, a and b in 2 limbs of 5 bits, radix
-    and grid tuned), against one launch per (kind, ring size) group in FP64
+    coefficients uniform in [-2^8, 2^8), seeded.  One persistent launch and
+    its combine launch replayed as a HIP graph (FP32 split, a and b in 2 limbs
+    of 5 bits; radix, grid, combine placement and LDS tile tuned), against
+    one launch per (kind, ring size) group in FP64
     (fused negacyclic plans; drop-in GPU_FFT C_C pairs for the linear
     products).  Rounded products of the persistent launch are checked
     against the FP64 launches."""
@@ -241,8 +242,11 @@ def batched_uneven(args):
     sid = s.cuda_stream
     plan = G.BatchedPlan(descs, 5, 2, 5, 2, stream=sid)
     with torch.cuda.stream(s):
-        cand, _ = A.tune_batched(plan, lambda: plan.execute(da, db, out, stream=sid), s,
-                                 key=A.signature("batched_uneven", count), use_db=False)
+        cand, timings = A.tune_batched(plan, lambda: plan.execute(da, db, out, stream=sid), s,
+                                       key=A.signature("batched_uneven", count), use_db=False)
+        best_per_tile = {}
+        for c, t in timings.items():
+            best_per_tile[c[3]] = min(best_per_tile.get(c[3], t), t)
         plan.execute(da, db, out, err, stream=sid)
         ms = A.time_ms(lambda: plan.graph_launch(da, db, out, stream=sid), s, reps=max(3, args.steps // 4))
     s.synchronize()
@@ -295,7 +299,10 @@ def batched_uneven(args):
     print(json.dumps({"workload": "batched uneven products", "count": count,
                       "ring_powers": "10..17", "kinds": "half linear (C_C), half negacyclic",
                       "coefficients": "[-2^8, 2^8)", "fp32_split": "a, b: 2 limbs of 5 bits",
-                      "tuned": list(cand), "persistent_graph_ms": ms,
+                      "tuned": list(cand),
+                      "tuned_fields": "radix bits, grid, combine in kernel, log2 LDS tile",
+                      "best_eager_ms_per_tile": {str(k): v for k, v in sorted(best_per_tile.items())},
+                      "persistent_graph_ms": ms,
                       "products_per_s": count / (ms * 1e-3), "alg_GBps": alg / (ms * 1e-3) / 1e9,
                       "max_err": float(err.max()), "fp64_per_size_ms": t64,
                       "size_groups": len(groups), "bit_exact_vs_fp64": bool(exact)}))

@@ -58,7 +58,7 @@ namespace {
 constexpr int KIND_NEGA = 0;
 constexpr int KIND_LINEAR = 1;
 constexpr int BT = 512;     // threads of the persistent workgroup
-constexpr int SMAX = 14;    // 2^14 float complex = 128 KiB of LDS
+constexpr int SMAX = 14;    // largest tile: 2^14 float complex = 128 KiB of LDS
 constexpr int SMIN = 5;
 constexpr int SBMAX = 4;
 constexpr int NQ = 8;       // work queues, one per XCD
@@ -259,17 +259,18 @@ __device__ __forceinline__ void stg_nt(gptr<cplx<float>> p, cplx<float> v)
     __builtin_nontemporal_store(w, (gptr<vec2>)p);
 }
 
-// The top SB inverse stages of position c < 2^SMAX of a split product over
-// all its limb-group sub-products in scratch, then the scale 1/L, the
-// untwist, the rounding of each group and the int64 recombination; writes
-// the unfolded outputs out[P], out[P + L] for P = j 2^SMAX + c.  Returns the
-// largest distance of a group output to its rounding.
+// The top SB inverse stages of position c < 2^S of a split product (S the
+// plan's tile) over all its limb-group sub-products in scratch, then the
+// scale 1/L, the untwist, the rounding of each group and the int64
+// recombination; writes the unfolded outputs out[P], out[P + L] for
+// P = j 2^S + c.  Returns the largest distance of a group output to its
+// rounding.
 template <int SB>
 __device__ __forceinline__ float combine_pos(const BArgs& ba, const BProd& pr, const BTables& tb,
                                              int c)
 {
     constexpr int Q = 1 << SB;
-    constexpr int S = SMAX;
+    const int S = pr.S;
     const int L = 1 << pr.n;
     const float scale = 1.0f / (float)L;
     const int ngrp = ba.sched->G;
@@ -361,14 +362,14 @@ __device__ __forceinline__ gptr<cplx<float>> stash_at(gptr<cplx<float>> base, in
 //      from its slot.
 // An inverse transform ends in emit(): rounding and accumulation into the
 // output (SB = 0) or the store of the sub-result to scratch (SB > 0).
-template <int KIND, int S, int E, int RB>
+template <int KIND, int S, int E, int RB, int TILE>
 __device__ __forceinline__ void run_item(cplx<float>* lds, const BArgs& ba, const BProd& pr,
                                          const BTables& tb, int prod, int q, int tid, int* sh_last)
 {
     constexpr int EPT = 1 << E;
     constexpr int NT = (1 << S) >> E;
-    constexpr int QMAX = S == SMAX ? 1 << SBMAX : 1;  // only S = SMAX items are split
-    const int SB = S == SMAX ? pr.SB : 0;
+    constexpr int QMAX = S == TILE ? 1 << SBMAX : 1;  // only S = TILE items are split
+    const int SB = S == TILE ? pr.SB : 0;
     const int Q = 1 << SB;
     constexpr int NST = (S + RB - 1) / RB;
     constexpr int NG = 1 << (E - RB);  // register groups of a thread
@@ -633,7 +634,7 @@ __device__ __forceinline__ void run_item(cplx<float>* lds, const BArgs& ba, cons
 // over the workgroup, but a call passes them in vector registers, where every
 // buffer load through a table pointer would become a loop over lanes: the
 // callee works on scalar copies of them.
-template <int KIND, int S, int E, int RB>
+template <int KIND, int S, int E, int RB, int TILE>
 __device__ __noinline__ void run_item_call(cplx<float>* lds, const BArgs& ba_in,
                                            const BProd& pr_in, const BTables& tb_in, int prod,
                                            int q, int tid, int* sh)
@@ -641,45 +642,50 @@ __device__ __noinline__ void run_item_call(cplx<float>* lds, const BArgs& ba_in,
     const BArgs ba = uniform_copy(ba_in);
     const BProd pr = uniform_copy(pr_in);
     const BTables tb = uniform_copy(tb_in);
-    run_item<KIND, S, E, RB>(lds, ba, pr, tb, __builtin_amdgcn_readfirstlane(prod),
+    run_item<KIND, S, E, RB, TILE>(lds, ba, pr, tb, __builtin_amdgcn_readfirstlane(prod),
                                  __builtin_amdgcn_readfirstlane(q), tid, sh);
 }
 
-template <int KIND, int S, int RB>
+template <int KIND, int S, int RB, int TILE>
 __device__ __forceinline__ void dispatch_sb(cplx<float>* lds, const BArgs& ba, const BProd& pr,
                                             const BTables& tb, int prod, int q, int tid, int* sh)
 {
-    constexpr int E = S >= 14 ? 5 : (S == 13 ? 4 : 3);  // NT <= 512
-    constexpr int R = RB < E ? RB : E;
-    if constexpr (S >= SMAX - 1)
-        run_item<KIND, S, E, R>(lds, ba, pr, tb, prod, q, tid, sh);
-    else
-        run_item_call<KIND, S, E, R>(lds, ba, pr, tb, prod, q, tid, sh);
+    if constexpr (S <= TILE) {
+        constexpr int E = S >= 14 ? 5 : (S == 13 ? 4 : 3);  // NT <= 512
+        constexpr int R = RB < E ? RB : E;
+        if constexpr (S >= TILE - 1)
+            run_item<KIND, S, E, R, TILE>(lds, ba, pr, tb, prod, q, tid, sh);
+        else
+            run_item_call<KIND, S, E, R, TILE>(lds, ba, pr, tb, prod, q, tid, sh);
+    }
 }
 
-template <int KIND, int RB>
+template <int KIND, int RB, int TILE>
 __device__ __forceinline__ void dispatch_s(cplx<float>* lds, const BArgs& ba, const BProd& pr,
                                            const BTables& tb, int prod, int q, int tid, int* sh)
 {
     switch (pr.S) {
-        case 5: dispatch_sb<KIND, 5, RB>(lds, ba, pr, tb, prod, q, tid, sh); break;
-        case 6: dispatch_sb<KIND, 6, RB>(lds, ba, pr, tb, prod, q, tid, sh); break;
-        case 7: dispatch_sb<KIND, 7, RB>(lds, ba, pr, tb, prod, q, tid, sh); break;
-        case 8: dispatch_sb<KIND, 8, RB>(lds, ba, pr, tb, prod, q, tid, sh); break;
-        case 9: dispatch_sb<KIND, 9, RB>(lds, ba, pr, tb, prod, q, tid, sh); break;
-        case 10: dispatch_sb<KIND, 10, RB>(lds, ba, pr, tb, prod, q, tid, sh); break;
-        case 11: dispatch_sb<KIND, 11, RB>(lds, ba, pr, tb, prod, q, tid, sh); break;
-        case 12: dispatch_sb<KIND, 12, RB>(lds, ba, pr, tb, prod, q, tid, sh); break;
-        case 13: dispatch_sb<KIND, 13, RB>(lds, ba, pr, tb, prod, q, tid, sh); break;
-        case 14: dispatch_sb<KIND, 14, RB>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 5: dispatch_sb<KIND, 5, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 6: dispatch_sb<KIND, 6, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 7: dispatch_sb<KIND, 7, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 8: dispatch_sb<KIND, 8, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 9: dispatch_sb<KIND, 9, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 10: dispatch_sb<KIND, 10, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 11: dispatch_sb<KIND, 11, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 12: dispatch_sb<KIND, 12, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 13: dispatch_sb<KIND, 13, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 14: dispatch_sb<KIND, 14, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
         default: break;
     }
 }
 
-template <int RB>
-__global__ __launch_bounds__(BT) void batched_kernel(BArgs ba)
+// TILE = 14: one 128 KiB workgroup per CU.  TILE = 13: 64 KiB workgroups,
+// compiled for 4 waves per SIMD so that two of them share a CU.
+template <int RB, int TILE>
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(TILE == 13 ? 4 : 1)))
+void batched_kernel(BArgs ba)
 {
-    __shared__ cplx<float> lds[1 << SMAX];
+    __shared__ cplx<float> lds[1 << TILE];
     __shared__ int sh_item;
     __shared__ int sh_last;
     const int tid = threadIdx.x;
@@ -702,16 +708,16 @@ __global__ __launch_bounds__(BT) void batched_kernel(BArgs ba)
             const BProd pr = uniform_copy(ba.prods[item.x]);
             const BTables tb = uniform_copy(ba.tabs[pr.tabset]);
             if (pr.kind == KIND_NEGA)
-                dispatch_s<KIND_NEGA, RB>(lds, ba, pr, tb, item.x, item.y, tid, &sh_last);
+                dispatch_s<KIND_NEGA, RB, TILE>(lds, ba, pr, tb, item.x, item.y, tid, &sh_last);
             else
-                dispatch_s<KIND_LINEAR, RB>(lds, ba, pr, tb, item.x, item.y, tid, &sh_last);
+                dispatch_s<KIND_LINEAR, RB, TILE>(lds, ba, pr, tb, item.x, item.y, tid, &sh_last);
             __syncthreads();
         }
     }
 }
 
-// One thread per position c < 2^SMAX of one split product (blockIdx.x: the
-// product's index in the split list).
+// One thread per position c < 2^S (S the tile) of one split product
+// (blockIdx.x: the product's index in the split list).
 __global__ __launch_bounds__(CT) void batched_combine_kernel(BArgs ba)
 {
     const int prod = __builtin_amdgcn_readfirstlane(ba.split[blockIdx.x]);
@@ -732,6 +738,9 @@ struct gpufft_batched_plan_s {
     int rb;
     int grid;
     int cmb;          // 1: combine split products in the persistent kernel
+    int tile;         // log2 of the LDS tile: 14 (128 KiB) or 13 (64 KiB, two per CU)
+    int max_n;        // largest log2 L of the list
+    long long scratch_elems;  // allocated scratch (cplx<float>)
     int qbeg[NQ], qlen[NQ];
     BProd* d_prods;
     int2* d_items;
@@ -839,6 +848,73 @@ int build_sched(int wa, int ka, int wb, int kb, BSched& sc)
     return (int)slot_free_after.size();
 }
 
+// Sub-problems, work queues, split list and scratch for LDS tile 2^tile:
+// each product is cut into 2^SB sub-problems of 2^S points, S = min(n, tile).
+// Products go largest first (cost ~ L log L), each to the least loaded queue;
+// a queue lists its products' sub-problems in that order.  Synchronises the
+// stream (the host vectors are freed on return).
+int build_layout(gpufft_batched_plan_s* p, int tile, hipStream_t st)
+{
+    const char* who = "gpufft_batched_plan: layout";
+    const int count = p->count;
+    long long scratch = 0;
+    for (BProd& pr : p->prods) {
+        pr.S = std::min(pr.n, tile);
+        pr.SB = pr.n - pr.S;
+        pr.scratch_off = 0;
+        if (pr.SB > 0) {
+            pr.scratch_off = scratch;
+            scratch += (long long)p->sched.G << pr.n;
+        }
+    }
+    auto cost = [&](int i) { return (long long)p->prods[i].n << p->prods[i].n; };
+    std::vector<int> order(count);
+    for (int i = 0; i < count; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost(x) > cost(y); });
+    std::vector<int> queue_of(count);
+    long long load[NQ] = {};
+    for (int i : order) {
+        const int x = (int)(std::min_element(load, load + NQ) - load);
+        queue_of[i] = x;
+        load[x] += cost(i);
+    }
+    std::vector<int2> items;
+    std::vector<int> split;
+    for (int x = 0; x < NQ; x++) {
+        p->qbeg[x] = (int)items.size();
+        for (int i : order)
+            if (queue_of[i] == x)
+                for (int q = 0; q < (1 << p->prods[i].SB); q++) items.push_back(make_int2(i, q));
+        p->qlen[x] = (int)items.size() - p->qbeg[x];
+    }
+    for (int i : order)
+        if (p->prods[i].SB > 0) split.push_back(i);
+    void* old[3] = {p->d_prods, p->d_items, p->d_split};
+    for (void* b : old)
+        if (b) (void)hipFree(b);
+    p->d_prods = nullptr;
+    p->d_items = nullptr;
+    p->d_split = nullptr;
+    int s;
+    if ((s = upload(&p->d_prods, p->prods, st)) || (s = upload(&p->d_items, items, st)) ||
+        (s = upload(&p->d_split, split, st)))
+        return s;
+    if (scratch > p->scratch_elems) {
+        if (p->scratch) (void)hipFree(p->scratch);
+        p->scratch = nullptr;
+        p->scratch_elems = 0;
+        hipError_t e = hipMalloc((void**)&p->scratch, sizeof(cplx<float>) * scratch);
+        if (e != hipSuccess) return fail_hip(e, who);
+        p->scratch_elems = scratch;
+    }
+    hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return fail_hip(e, who);
+    p->n_items = (int)items.size();
+    p->n_split = (int)split.size();
+    p->tile = tile;
+    return GPUFFT_OK;
+}
+
 int alloc_stash(gpufft_batched_plan_s* p, int grid)
 {
     if (p->stash_slots == 0 || grid <= p->stash_grid) return GPUFFT_OK;
@@ -896,14 +972,23 @@ int launch(gpufft_batched_plan_s* p, const int* a, const int* b, long long* out,
     ba.oa = p->ka > 1 ? offset(p->wa, p->ka) : 0;
     ba.ob = p->kb > 1 ? offset(p->wb, p->kb) : 0;
     ba.err = err;
-    switch (p->rb) {
-        case 3: hipLaunchKernelGGL(batched_kernel<3>, dim3(p->grid), dim3(BT), 0, st, ba); break;
-        case 4: hipLaunchKernelGGL(batched_kernel<4>, dim3(p->grid), dim3(BT), 0, st, ba); break;
-        default: hipLaunchKernelGGL(batched_kernel<5>, dim3(p->grid), dim3(BT), 0, st, ba); break;
+    const dim3 grid(p->grid), block(BT);
+    if (p->tile == 13) {
+        switch (p->rb) {
+            case 3: hipLaunchKernelGGL((batched_kernel<3, 13>), grid, block, 0, st, ba); break;
+            case 4: hipLaunchKernelGGL((batched_kernel<4, 13>), grid, block, 0, st, ba); break;
+            default: hipLaunchKernelGGL((batched_kernel<5, 13>), grid, block, 0, st, ba); break;
+        }
+    } else {
+        switch (p->rb) {
+            case 3: hipLaunchKernelGGL((batched_kernel<3, 14>), grid, block, 0, st, ba); break;
+            case 4: hipLaunchKernelGGL((batched_kernel<4, 14>), grid, block, 0, st, ba); break;
+            default: hipLaunchKernelGGL((batched_kernel<5, 14>), grid, block, 0, st, ba); break;
+        }
     }
     GFA_CHECK_LAUNCH(who);
     if (!p->cmb && p->n_split > 0) {
-        hipLaunchKernelGGL(batched_combine_kernel, dim3(p->n_split, (1 << SMAX) / CT), dim3(CT), 0,
+        hipLaunchKernelGGL(batched_combine_kernel, dim3(p->n_split, (1 << p->tile) / CT), dim3(CT), 0,
                            st, ba);
         GFA_CHECK_LAUNCH(who);
     }
@@ -945,13 +1030,11 @@ int gpufft_batched_plan_create(gpufft_batched_plan* out, const gpufft_product_de
 
     std::map<int, int> tabset;  // n -> table set (both kinds use the FFNT tables of L = 2^n)
     std::vector<BTables> tabs;
-    long long scratch = 0;
     // group slots after the kb - 1 stashed b-limb spectra
     const int gslots = build_sched(p->wa, p->ka, p->wb, p->kb, p->sched);
     for (int g = 0; g < p->sched.G; g++)
         if (p->sched.gslot[g] >= 0) p->sched.gslot[g] += p->kb - 1;
     p->stash_slots = p->kb - 1 + gslots;
-    const int ngrp = p->sched.G;
     for (int i = 0; i < count; i++) {
         const gpufft_product_desc& d = descs[i];
         BProd pr{};
@@ -975,8 +1058,6 @@ int gpufft_batched_plan_create(gpufft_batched_plan* out, const gpufft_product_de
             return fail(GPUFFT_ERR_INVALID_ARGUMENT,
                         std::string(who) + ": a_offset and b_offset must be multiples of 4");
         }
-        pr.S = std::min(pr.n, SMAX);
-        pr.SB = pr.n - pr.S;
         pr.a_off = d.a_offset;
         pr.b_off = d.b_offset;
         pr.N = 1 << d.ring_power;
@@ -1010,49 +1091,17 @@ int gpufft_batched_plan_create(gpufft_batched_plan* out, const gpufft_product_de
             tabs.push_back(t);
         }
         pr.tabset = f->second;
-        if (pr.SB > 0) {
-            pr.scratch_off = scratch;
-            scratch += (long long)ngrp << pr.n;
-        }
+        p->max_n = std::max(p->max_n, pr.n);
         p->prods.push_back(pr);
     }
-    // products largest first (cost ~ L log L), each dealt to the least loaded
-    // queue; a queue lists its products' sub-problems in that order
-    auto cost = [&](int i) { return (long long)p->prods[i].n << p->prods[i].n; };
-    std::vector<int> order(count);
-    for (int i = 0; i < count; i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost(x) > cost(y); });
-    std::vector<int> queue_of(count);
-    long long load[NQ] = {};
-    for (int i : order) {
-        const int x = (int)(std::min_element(load, load + NQ) - load);
-        queue_of[i] = x;
-        load[x] += cost(i);
-    }
-    std::vector<int2> items;
-    std::vector<int> split;
-    for (int x = 0; x < NQ; x++) {
-        p->qbeg[x] = (int)items.size();
-        for (int i : order)
-            if (queue_of[i] == x)
-                for (int q = 0; q < (1 << p->prods[i].SB); q++) items.push_back(make_int2(i, q));
-        p->qlen[x] = (int)items.size() - p->qbeg[x];
-    }
-    for (int i : order)
-        if (p->prods[i].SB > 0) split.push_back(i);
-    p->n_items = (int)items.size();
-    p->n_split = (int)split.size();
     const std::vector<BSched> schedv(1, p->sched);
     int s;
-    if ((s = upload(&p->d_prods, p->prods, st)) || (s = upload(&p->d_items, items, st)) ||
-        (s = upload(&p->d_tabs, tabs, st)) || (s = upload(&p->d_split, split, st)) ||
-        (s = upload(&p->d_sched, schedv, st)) ||
-        (s = alloc_stash(p, p->grid))) {
+    if ((s = upload(&p->d_tabs, tabs, st)) || (s = upload(&p->d_sched, schedv, st)) ||
+        (s = build_layout(p, SMAX, st)) || (s = alloc_stash(p, p->grid))) {
         release(p);
         return s;
     }
     hipError_t e = hipMalloc((void**)&p->d_counters, sizeof(int) * (count + NQ));
-    if (e == hipSuccess && scratch > 0) e = hipMalloc((void**)&p->scratch, sizeof(cplx<float>) * scratch);
     if (e == hipSuccess) e = hipMalloc(&p->wc, sizeof(cplx<float>) * 32);
     if (e == hipSuccess) e = hipMalloc(&p->wci, sizeof(cplx<float>) * 32);
     if (e != hipSuccess) {
@@ -1109,6 +1158,32 @@ int gpufft_batched_plan_set_combine(gpufft_batched_plan plan, int in_kernel)
     return ok();
 }
 
+int gpufft_batched_plan_set_tile(gpufft_batched_plan plan, int log2_tile)
+{
+    const char* who = "gpufft_batched_plan_set_tile";
+    if (!plan) return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": plan is NULL");
+    if (log2_tile != 13 && log2_tile != 14)
+        return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": log2_tile must be 13 or 14");
+    if (plan->max_n - log2_tile > SBMAX)
+        return fail(GPUFFT_ERR_INVALID_ARGUMENT,
+                    std::string(who) + ": the largest product needs more than 2^4 sub-problems");
+    if (log2_tile == plan->tile) return ok();
+    if (plan->graph) {
+        (void)hipGraphExecDestroy(plan->graph);
+        plan->graph = nullptr;
+    }
+    if (int s = build_layout(plan, log2_tile, nullptr)) return s;
+    return ok();
+}
+
+int gpufft_batched_plan_get_tile(gpufft_batched_plan plan, int* log2_tile)
+{
+    const char* who = "gpufft_batched_plan_get_tile";
+    if (!plan || !log2_tile) return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": NULL argument");
+    *log2_tile = plan->tile;
+    return ok();
+}
+
 int gpufft_batched_plan_get_combine(gpufft_batched_plan plan, int* in_kernel)
 {
     const char* who = "gpufft_batched_plan_get_combine";
@@ -1146,6 +1221,9 @@ int gpufft_batched_graph_launch(gpufft_batched_plan plan, const int32_t* a, cons
     const char* who = "gpufft_batched_graph_launch";
     if (!plan) return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": plan is NULL");
     if (!a || !b || !out) return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": NULL buffer");
+    if (!stream)
+        return fail(GPUFFT_ERR_INVALID_ARGUMENT,
+                    std::string(who) + ": graph capture needs a stream other than the default stream");
     hipStream_t st = (hipStream_t)stream;
     if (!plan->graph || plan->g_a != a || plan->g_b != b || plan->g_out != out ||
         plan->g_err != err_stats || plan->g_stream != stream) {

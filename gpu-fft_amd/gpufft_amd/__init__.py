@@ -86,6 +86,8 @@ _SIGS = {
     "gpufft_batched_plan_set_tuning": [_P, _I, _I],
     "gpufft_batched_plan_set_combine": [_P, _I],
     "gpufft_batched_plan_get_combine": [_P, _P],
+    "gpufft_batched_plan_set_tile": [_P, _I],
+    "gpufft_batched_plan_get_tile": [_P, _P],
     "gpufft_batched_plan_info": [_P, _P, _P, _P, _P],
     "gpufft_batched_execute": [_P, _P, _P, _P, _P, _P],
     "gpufft_batched_graph_launch": [_P, _P, _P, _P, _P, _P],
@@ -365,6 +367,17 @@ class BatchedPlan:
     @combine_in_kernel.setter
     def combine_in_kernel(self, on):
         check(_lib.gpufft_batched_plan_set_combine(self.handle, 1 if on else 0))
+
+    @property
+    def tile(self):
+        """log2 of the LDS tile (points per workgroup): 14 or 13."""
+        v = ctypes.c_int()
+        check(_lib.gpufft_batched_plan_get_tile(self.handle, ctypes.byref(v)))
+        return v.value
+
+    @tile.setter
+    def tile(self, log2_tile):
+        check(_lib.gpufft_batched_plan_set_tile(self.handle, int(log2_tile)))
 
     def info(self):
         n, s, rb, g = ctypes.c_int(), ctypes.c_longlong(), ctypes.c_int(), ctypes.c_int()

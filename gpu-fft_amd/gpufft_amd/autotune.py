@@ -115,15 +115,31 @@ def tune_polymul(plan, run, stream, key=None, reps=5, use_db=True, db_file=None)
                 reps, use_db, db_file)
 
 
-def tune_batched(plan, run, stream, key, grids=None, reps=3, use_db=True, db_file=None):
-    """Radix of the LDS steps, persistent grid size and where split products
-    are combined (0: separate kernel, 1: inside the persistent kernel)."""
+def tune_batched(plan, run, stream, key, grids=None, reps=3, use_db=True, db_file=None,
+                 tiles=(14, 13)):
+    """Radix of the LDS steps, persistent grid size, where split products are
+    combined (0: separate kernel, 1: inside the persistent kernel) and the
+    LDS tile (log2 points per workgroup: 14, one workgroup of 8 waves per
+    CU, or 13, two per CU).  Tiles the plan refuses (a product would need
+    more than 2^4 sub-problems) are skipped."""
     import torch
     cus = torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
-    grids = grids or (cus // 2, cus)
-    cands = [(rb, g, cmb) for cmb in (0, 1) for rb in (3, 4, 5) for g in grids]
+    ok_tiles = []
+    for t in tiles:
+        try:
+            plan.tile = t
+            ok_tiles.append(t)
+        except ValueError:
+            pass
+    cands = []
+    for t in ok_tiles:
+        per_cu = 1 if t == 14 else 2
+        gs = grids or (cus * per_cu // 2, cus * per_cu)
+        cands += [(rb, g, cmb, t) for cmb in (0, 1) for rb in (3, 4, 5) for g in gs]
 
     def apply(c):
+        if len(c) > 3 and plan.tile != c[3]:
+            plan.tile = c[3]
         plan.set_tuning(c[0], c[1])
         plan.combine_in_kernel = bool(c[2]) if len(c) > 2 else False
     return tune(key, cands, apply, run, stream, reps, use_db, db_file)

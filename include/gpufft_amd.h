@@ -253,7 +253,8 @@ int gpufft_ckks_decode_f64(const int64_t* coeffs, void* slots, const void* table
  * of the product to finish (gpufft_batched_plan_set_combine(plan, 1)).
  * gpufft_batched_execute enqueues the launches on `stream`;
  * gpufft_batched_graph_launch captures them into a HIP graph on first use
- * (per buffer set) and replays the graph.  err_stats
+ * per buffer set and replays the graph; it needs a stream other than the
+ * default stream, which cannot be captured.  err_stats
  * (optional, double [count]) receives each product's largest distance of a
  * limb output to its rounding (< 0.5: every limb was rounded exactly).
  * Calls on one plan must not overlap. */
@@ -276,6 +277,12 @@ int gpufft_batched_plan_set_tuning(gpufft_batched_plan plan, int radix_bits, int
  * 1 = in the persistent kernel (one launch); chosen by the autotuner */
 int gpufft_batched_plan_set_combine(gpufft_batched_plan plan, int in_kernel);
 int gpufft_batched_plan_get_combine(gpufft_batched_plan plan, int* in_kernel);
+/* LDS tile of a workgroup, log2 of its points: 14 (128 KiB, one workgroup
+ * of 8 waves per CU; default) or 13 (64 KiB, two per CU; products above
+ * the tile are split into more sub-problems, at most 2^4).  Rebuilds the
+ * work list and synchronises; chosen by the autotuner */
+int gpufft_batched_plan_set_tile(gpufft_batched_plan plan, int log2_tile);
+int gpufft_batched_plan_get_tile(gpufft_batched_plan plan, int* log2_tile);
 int gpufft_batched_plan_info(gpufft_batched_plan plan, int* n_items, long long* scratch_elems,
                              int* radix_bits, int* grid);
 int gpufft_batched_execute(gpufft_batched_plan plan, const int32_t* a, const int32_t* b,

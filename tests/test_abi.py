@@ -79,6 +79,8 @@ def test_polymul_plan_validation_on_host():
     assert G.lib().gpufft_polymul_plan_get_a_center(None) == -1
     assert G.lib().gpufft_batched_plan_set_combine(None, 1) == G.ERR_NULL_POINTER
     assert G.lib().gpufft_batched_plan_get_combine(None, None) == G.ERR_NULL_POINTER
+    assert G.lib().gpufft_batched_plan_set_tile(None, 13) == G.ERR_NULL_POINTER
+    assert G.lib().gpufft_batched_plan_get_tile(None, None) == G.ERR_NULL_POINTER
 
 
 def test_table_generation_validation_on_host():

@@ -139,3 +139,48 @@ def test_batched_limb_schedules(split):
         e = to_host(err)
         assert np.all(e < 0.5), (split, e)
         assert np.array_equal(to_host(out), exact), (split, in_kernel)
+
+
+@pytest.mark.parametrize("rb", [3, 4, 5])
+def test_batched_tile_13(rb):
+    """The 64 KiB tile (two workgroups per CU; products above 2^13 points
+    split into more sub-problems): exact for both combine placements, and
+    switching the tile back and forth rebuilds the work list correctly."""
+    import torch
+
+    import gpufft_amd as G
+    kinds, rps = [1, 0, 0, 1, 1, 0], [17, 17, 14, 13, 9, 12]
+    descs, a, b, exact = _workload(kinds, rps, 0x6770756666740144)
+    plan = G.BatchedPlan(descs, **LIMBS)
+    da, db = to_dev(a.astype(np.int32)), to_dev(b.astype(np.int32))
+    items14 = plan.info()["items"]
+    plan.tile = 13
+    assert plan.tile == 13 and plan.info()["items"] > items14
+    plan.set_tuning(rb, 512)
+    for in_kernel in (False, True):
+        plan.combine_in_kernel = in_kernel
+        out = torch.full((len(exact),), -1, dtype=torch.int64, device="cuda")
+        err = torch.zeros(len(descs), dtype=torch.float64, device="cuda")
+        plan.execute(da, db, out, err)
+        assert np.all(to_host(err) < 0.5)
+        assert np.array_equal(to_host(out), exact), in_kernel
+    plan.tile = 14
+    assert plan.info()["items"] == items14
+    out = torch.zeros(len(exact), dtype=torch.int64, device="cuda")
+    with pytest.raises(ValueError, match="default stream"):
+        plan.graph_launch(da, db, out)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        plan.graph_launch(da, db, out, stream=s)
+    s.synchronize()
+    assert np.array_equal(to_host(out), exact)
+
+
+def test_batched_tile_refused_for_large_products():
+    """A ring that would need 2^5 sub-problems at the 2^13 tile keeps 2^14."""
+    import gpufft_amd as G
+    descs, _, _, _ = _workload([0], [19], 0x6770756666740184)
+    plan = G.BatchedPlan(descs, **LIMBS)
+    with pytest.raises(ValueError, match="sub-problems"):
+        plan.tile = 13
+    assert plan.tile == 14
