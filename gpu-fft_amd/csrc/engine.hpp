@@ -242,7 +242,6 @@ __device__ __forceinline__ void group_stages_derived(cplx<T>* v, const PassArgs<
                                                      const cplx<T>* __restrict__ wc)
 {
     static_assert(!mode_special(MODE), "derived twiddles are for the standard modes");
-    constexpr int GS = 1 << RHO;
 #pragma unroll
     for (int st = 0; st < RHO; st++) {
         const int i = mode_descending(MODE) ? (RHO - 1 - st) : st;
@@ -302,7 +301,8 @@ __device__ __forceinline__ void radix_step(cplx<T>* lds, const PassArgs<T>& a, c
     constexpr int GPT = (1 << E) / GS;
     // with derived twiddles (fused kernels, tight on registers) the groups
     // of a thread are processed one after the other
-#pragma unroll(DRV ? 1 : GPT)
+    constexpr int UNROLL = DRV ? 1 : GPT;
+#pragma unroll UNROLL
     for (int k = 0; k < GPT; k++) {
         int rbase, col;
         group_of<RHO, NT>(tid, k, g.logC, q, rbase, col);
