@@ -477,6 +477,17 @@ def config_suite(args):
                           "f64_round_trip_max_slot_err": e_rt64}), flush=True)
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def tune_radix(plan, candidates, fn, stream, steps):
     """Radix bits of the product kernel: every candidate timed in three
     interleaved rounds, the minimum kept (one round is noisy)."""
@@ -645,7 +656,7 @@ def main():
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
-    cpu = None
+    cpu = cpu_par = None
     if not args.no_cpu:
         import oracle as O
         na, nb = a.reshape(batch, N), b.reshape(batch, N)
@@ -657,9 +668,25 @@ def main():
             if time.perf_counter() - t0 > args.cpu_seconds:
                 break
         dt = time.perf_counter() - t0
+        model = _cpu_model()
         cpu = {"value": cnt / dt, "unit": "poly-mults/s", "cores": 1, "kind": "port",
                "sample": "%d products of N=2^%d (FFNT<double>::MULT restated in C, one thread, "
-                         "%.1f s)" % (cnt, rp, dt)}
+                         "%.1f s)" % (cnt, rp, dt), "cpu_model": model}
+        # batch-parallel: one product per task over the host cores available
+        # to this process (at most 16, the host share of one GPU here)
+        P = max(1, min(16, len(os.sched_getaffinity(0))))
+        m = min(batch, 4 * P)
+        t0 = time.perf_counter()
+        cnt = 0
+        while True:
+            O.cpu_ffnt_mult(na[:m], nb[:m], threads=P)
+            cnt += m
+            if time.perf_counter() - t0 > args.cpu_seconds / 2:
+                break
+        dt = time.perf_counter() - t0
+        cpu_par = {"value": cnt / dt, "unit": "poly-mults/s", "cores": P, "kind": "port",
+                   "sample": "%d products of N=2^%d (the same C restatement, %d threads, one "
+                             "product per task, %.1f s)" % (cnt, rp, P, dt), "cpu_model": model}
 
     line = {
         "metric": METRIC,
@@ -681,6 +708,7 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic},
         "cpu_baseline": cpu,
+        "cpu_baseline_parallel": cpu_par,
         "forms": res,
     }
     print(json.dumps(line))
