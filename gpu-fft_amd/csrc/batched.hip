@@ -13,8 +13,10 @@
 //               length 2N (example/test_fft_gpu_C_C.cu) at half its
 //               transform length.
 //
-// Every product is cut into 2^SB sub-problems of 2^S points (S = min(log2 L,
-// 14)) as in fused.hpp; the work list holds one item per sub-problem.  The
+// Every product is cut into 2^SB sub-problems of 2^S points, S = min(log2 L,
+// T), as in fused.hpp; T is the plan's LDS tile, 14 (128 KiB, one workgroup
+// per CU) or 13 (64 KiB, two per CU).  The work list holds one item per
+// sub-problem.  The
 // list is split into 8 queues, one per XCD: the hardware deals the workgroups
 // of the persistent grid round-robin to the 8 XCDs, and workgroup b serves
 // queue b mod 8 first.  All sub-problems of a product sit in one queue, so
@@ -29,7 +31,8 @@
 // one inverse transform; their magnitudes add, which the error statistics
 // see.  Spectra and partial sums that do not stay in registers live in a
 // small per-workgroup stash.  For a and b in 2 limbs each that is 4 forward
-// and 3 inverse transforms instead of 4 products of 2 forward and 1 inverse.
+// and 3 inverse transforms, against 10 when each limb product is formed on
+// its own (b limb, then every a limb again, then the inverse).
 //
 // An item with SB = 0 is a whole product: it rounds each group and
 // accumulates it into the output.  An item with SB > 0 writes its group
