@@ -165,20 +165,6 @@ template <typename T> __device__ __forceinline__ gptr<T> to_global(T* p)
     return (gptr<T>)p;
 }
 
-// Balanced base-2^w digit i of an int32 v in 32-bit arithmetic (as
-// limb_digit): with o = sum_{j < k-1} 2^(w-1) 2^(w j) < 2^s, s = w (k-1) <= 30,
-// the digits below the top are bits of v + o mod 2^32, and the top digit
-// floor((v + o) / 2^s) = (v >> s) + (((v mod 2^s) + o) >> s), whose second
-// sum stays below 2^31.
-__device__ __forceinline__ int limb_digit32(int v, int w, int i, int k, int o)
-{
-    if (k == 1) return v;
-    const int s = w * (k - 1);
-    if (i == k - 1) return (v >> s) + (((v & ((1 << s) - 1)) + o) >> s);
-    const unsigned u = (unsigned)v + (unsigned)o;
-    return (int)((u >> (w * i)) & ((1u << w) - 1)) - (1 << (w - 1));
-}
-
 // Sub-problem q of one limb of one operand into LDS: for each of the
 // workgroup's 2^S positions c, row q of the top SB forward stages over the
 // folded inputs at j 2^S + c (Q = 2^SB partners), times the twist
@@ -227,9 +213,9 @@ __device__ __forceinline__ void item_load(cplx<float>* lds, gptr<const cplx<floa
                 float ax = 0.f, ay = 0.f;
 #pragma unroll
                 for (int j = 0; j < Q; j++) {
-                    const float re = (float)limb_digit32(xr[u][j][v], w, limb, k, o);
+                    const float re = limb_digit(xr[u][j][v], w, limb, k, o, 0);
                     if constexpr (KIND == KIND_NEGA) {
-                        const float im = (float)limb_digit32(xi[u][j][v], w, limb, k, o);
+                        const float im = limb_digit(xi[u][j][v], w, limb, k, o, 0);
                         ax = ax + C[j].x * re - C[j].y * im;
                         ay = ay + C[j].x * im + C[j].y * re;
                     } else {

@@ -64,16 +64,21 @@ template <typename T> struct ProdArgs {
     int* arrive;           // per-product arrival counters (in-kernel combine)
 };
 
-// Balanced base-2^w digit i of v (digits d_0 .. d_{k-1}, the top one keeps
-// the rest): with o = sum_{j < k-1} 2^(w-1) 2^(w j), the unsigned base-2^w
-// digits u_j of v + o give d_j = u_j - 2^(w-1) for j < k-1, and the top digit
-// is (v + o) >> (w (k-1)) (floor).
-__device__ __forceinline__ int limb_digit(long long v, int w, int i, int k, long long o)
+// Balanced base-2^w digit i of x = v - center, v an int32 (digits d_0 ..
+// d_{k-1}, the top one keeps the rest): with o = sum_{j < k-1} 2^(w-1) 2^(w j),
+// the unsigned base-2^w digits u_j of x + o give d_j = u_j - 2^(w-1) for
+// j < k-1, and the top digit is floor((x + o) / 2^s), s = w (k-1).  In 32-bit
+// arithmetic, exact for every int32 v when s <= 30 and center <= 2^30 (the
+// plans check both): the digits below the top are bits of v - center + o
+// mod 2^32, and the top digit is (v >> s) + (((v mod 2^s) - center + o) >> s),
+// whose second sum lies in (-2^31, 2^31).
+__device__ __forceinline__ float limb_digit(int v, int w, int i, int k, int o, int center)
 {
-    if (k == 1) return (int)v;
-    const long long u = v + o;
-    if (i == k - 1) return (int)(u >> (w * (k - 1)));
-    return (int)((u >> (w * i)) & ((1LL << w) - 1)) - (1 << (w - 1));
+    if (k == 1) return (float)((long long)v - center);
+    const int s = w * (k - 1);
+    if (i == k - 1) return (float)((v >> s) + (((v & ((1 << s) - 1)) - center + o) >> s));
+    const unsigned u = (unsigned)v - (unsigned)center + (unsigned)o;
+    return (float)((int)((u >> (w * i)) & ((1u << w) - 1)) - (1 << (w - 1)));
 }
 
 // Positions per load of an operand: 16-byte loads of 2 doubles (FP64 form);
@@ -104,16 +109,16 @@ __device__ __forceinline__ void prod_input_vec(rsrc_t x, int L, int w, int limb,
         const u32x4 i = __builtin_amdgcn_raw_buffer_load_b128(x, (unsigned)(P + L) * 4u, 0, 0);
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            re[v] = (T)limb_digit((long long)(int)r[v] - center, w, limb, k, o);
-            im[v] = (T)limb_digit((long long)(int)i[v] - center, w, limb, k, o);
+            re[v] = (T)limb_digit((int)r[v], w, limb, k, (int)o, center);
+            im[v] = (T)limb_digit((int)i[v], w, limb, k, (int)o, center);
         }
     } else {
         const u32x2 r = __builtin_amdgcn_raw_buffer_load_b64(x, (unsigned)P * 4u, 0, 0);
         const u32x2 i = __builtin_amdgcn_raw_buffer_load_b64(x, (unsigned)(P + L) * 4u, 0, 0);
 #pragma unroll
         for (int v = 0; v < 2; v++) {
-            re[v] = (T)limb_digit((long long)(int)r[v] - center, w, limb, k, o);
-            im[v] = (T)limb_digit((long long)(int)i[v] - center, w, limb, k, o);
+            re[v] = (T)limb_digit((int)r[v], w, limb, k, (int)o, center);
+            im[v] = (T)limb_digit((int)i[v], w, limb, k, (int)o, center);
         }
     }
 }

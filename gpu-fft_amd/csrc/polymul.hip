@@ -237,7 +237,9 @@ int gpufft_polymul_plan_create(gpufft_polymul_plan* out, int ring_power, int pre
     if (precision == GPUFFT_PREC_F32_SPLIT) {
         if (a_limbs < 1 || b_limbs < 1 || a_limbs * b_limbs > 64 ||
             (a_limbs > 1 && (a_limb_bits < 1 || a_limb_bits > 16)) ||
-            (b_limbs > 1 && (b_limb_bits < 1 || b_limb_bits > 16)))
+            (b_limbs > 1 && (b_limb_bits < 1 || b_limb_bits > 16)) ||
+            (a_limbs > 1 && (a_limbs - 1) * a_limb_bits > 30) ||
+            (b_limbs > 1 && (b_limbs - 1) * b_limb_bits > 30))
             return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": invalid limb split");
     }
     gpufft_polymul_plan_s* pl = new gpufft_polymul_plan_s();

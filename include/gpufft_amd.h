@@ -151,7 +151,9 @@ int gpufft_generate_table_f32(void* dst, int kind, int n_power, int inverse, voi
  *   GPUFFT_PREC_F32_SPLIT a, b: int32 [batch][N]; out: int64 [batch][N], the
  *                         product mod X^N+1.  a is split into a_limbs
  *                         balanced base-2^a_limb_bits digits (the top one
- *                         keeps the rest), b into b_limbs digits; each limb
+ *                         keeps the rest), b into b_limbs digits (the top
+ *                         digit must start at bit 30 or below:
+ *                         (limbs - 1) * limb_bits <= 30); each limb
  *                         product is an FP32 FFNT product rounded to an
  *                         integer, and the limbs are recombined in int64.
  *                         err_stats (optional, double [batch][2]) receives

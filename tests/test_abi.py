@@ -71,6 +71,8 @@ def test_polymul_plan_validation_on_host():
             G.PolymulPlan(*args, stream=0)
     with pytest.raises(ValueError, match="limb"):
         G.PolymulPlan(12, G.PREC_F32_SPLIT, 4, 40, 2, 0, 1, stream=0)
+    with pytest.raises(ValueError, match="limb"):  # top digit above bit 30
+        G.PolymulPlan(12, G.PREC_F32_SPLIT, 4, 16, 3, 0, 1, stream=0)
     with pytest.raises(ValueError, match="max_batch"):
         G.PolymulPlan(12, G.PREC_F64, 0, stream=0)
     assert G.lib().gpufft_polymul_plan_set_combine(None, 1) == G.ERR_NULL_POINTER
