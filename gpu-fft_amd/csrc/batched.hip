@@ -306,6 +306,76 @@ __device__ __forceinline__ float combine_pos(const BArgs& ba, const BProd& pr, c
     return em;
 }
 
+// combine_pos for the two adjacent positions c, c + 1 (c even): one 16-byte
+// load per row, group and untwist pair.
+template <int SB>
+__device__ __forceinline__ float combine_pos2(const BArgs& ba, const BProd& pr, const BTables& tb,
+                                              int c)
+{
+    typedef float f32x4 __attribute__((ext_vector_type(4)));
+    constexpr int Q = 1 << SB;
+    const int S = pr.S;
+    const int L = 1 << pr.n;
+    const float scale = 1.0f / (float)L;
+    const int ngrp = ba.sched->G;
+    long long acc[Q][2][2];  // [row][position][re, im]
+#pragma unroll
+    for (int j = 0; j < Q; j++) acc[j][0][0] = acc[j][0][1] = acc[j][1][0] = acc[j][1][1] = 0;
+    float em = 0.f;
+    const gptr<const cplx<float>> untwist = to_global(tb.untwist);
+    for (int l = 0; l < ngrp; l++) {
+        const gptr<const cplx<float>> src = to_global(ba.scratch + pr.scratch_off + (long long)l * L);
+        cplx<float> y[2][Q];
+#pragma unroll
+        for (int j = 0; j < Q; j++) {
+            const f32x4 w = __builtin_nontemporal_load((gptr<const f32x4>)(src + (j << S) + c));
+            y[0][j] = mk<float>(w[0], w[1]);
+            y[1][j] = mk<float>(w[2], w[3]);
+        }
+#pragma unroll
+        for (int i = 0; i < SB; i++) {  // Gentleman-Sande, bits S .. n-1
+#pragma unroll
+            for (int j0 = 0; j0 < Q; j0++) {
+                if (j0 & (1 << i)) continue;
+                const int j1 = j0 | (1 << i);
+                const cplx<float> w = tb.iroot[j0 >> (i + 1)];
+#pragma unroll
+                for (int v = 0; v < 2; v++) {
+                    const cplx<float> U = y[v][j0], V = y[v][j1];
+                    y[v][j0] = U + V;
+                    y[v][j1] = (U - V) * w;
+                }
+            }
+        }
+        const long long wgt = 1LL << ba.sched->gshift[l];
+#pragma unroll
+        for (int j = 0; j < Q; j++) {
+            const int P = (j << S) | c;
+            const f32x4 ut = *(gptr<const f32x4>)(untwist + P);
+#pragma unroll
+            for (int v = 0; v < 2; v++) {
+                const cplx<float> u = v ? mk<float>(ut[2], ut[3]) : mk<float>(ut[0], ut[1]);
+                const cplx<float> z = mk<float>(y[v][j].x * scale, y[v][j].y * scale) * u;
+                const float r0 = rintf(z.x), r1 = rintf(z.y);
+                em = fmaxf(em, fmaxf(fabsf(z.x - r0), fabsf(z.y - r1)));
+                acc[j][v][0] += (long long)r0 * wgt;
+                acc[j][v][1] += (long long)r1 * wgt;
+            }
+        }
+    }
+    const gptr<long long> o = to_global(ba.out + pr.out_off);
+#pragma unroll
+    for (int j = 0; j < Q; j++) {
+        const int P = (j << S) | c;
+#pragma unroll
+        for (int v = 0; v < 2; v++) {
+            __builtin_nontemporal_store(acc[j][v][0], o + P + v);
+            __builtin_nontemporal_store(acc[j][v][1], o + P + v + L);
+        }
+    }
+    return em;
+}
+
 __device__ __forceinline__ float combine_pos_rt(const BArgs& ba, const BProd& pr,
                                                 const BTables& tb, int c)
 {
@@ -705,15 +775,23 @@ void batched_kernel(BArgs ba)
     }
 }
 
-// One thread per position c < 2^S (S the tile) of one split product
-// (blockIdx.x: the product's index in the split list).
+// One thread per pair of adjacent positions c, c + 1 < 2^S (S the tile) of
+// one split product (blockIdx.x: the product's index in the split list).
 __global__ __launch_bounds__(CT) void batched_combine_kernel(BArgs ba)
 {
     const int prod = __builtin_amdgcn_readfirstlane(ba.split[blockIdx.x]);
     const BProd pr = uniform_copy(ba.prods[prod]);
     const BTables tb = uniform_copy(ba.tabs[pr.tabset]);
-    const int c = blockIdx.y * CT + threadIdx.x;
-    const float em = combine_pos_rt(ba, pr, tb, c);
+    const int c = (blockIdx.y * CT + threadIdx.x) * 2;
+    float em;
+    switch (pr.SB) {
+        case 1: em = combine_pos2<1>(ba, pr, tb, c); break;
+        case 2: em = combine_pos2<2>(ba, pr, tb, c); break;
+        case 3: em = combine_pos2<3>(ba, pr, tb, c); break;
+        default:  // 2^4 rows: one position at a time (registers)
+            em = fmaxf(combine_pos<4>(ba, pr, tb, c), combine_pos<4>(ba, pr, tb, c + 1));
+            break;
+    }
     err_update(ba.err, prod, em);
 }
 
@@ -977,8 +1055,8 @@ int launch(gpufft_batched_plan_s* p, const int* a, const int* b, long long* out,
     }
     GFA_CHECK_LAUNCH(who);
     if (!p->cmb && p->n_split > 0) {
-        hipLaunchKernelGGL(batched_combine_kernel, dim3(p->n_split, (1 << p->tile) / CT), dim3(CT), 0,
-                           st, ba);
+        hipLaunchKernelGGL(batched_combine_kernel, dim3(p->n_split, (1 << p->tile) / (2 * CT)),
+                           dim3(CT), 0, st, ba);
         GFA_CHECK_LAUNCH(who);
     }
     return GPUFFT_OK;
