@@ -673,7 +673,7 @@ def main():
                "sample": "%d products of N=2^%d (FFNT<double>::MULT restated in C, one thread, "
                          "%.1f s)" % (cnt, rp, dt), "cpu_model": model}
         # batch-parallel: one product per task over the host cores available
-        # to this process (at most 16, the host share of one GPU here)
+        # to this process, at most 16 threads
         P = max(1, min(16, len(os.sched_getaffinity(0))))
         m = min(batch, 4 * P)
         t0 = time.perf_counter()
