@@ -184,3 +184,29 @@ def test_batched_tile_refused_for_large_products():
     with pytest.raises(ValueError, match="sub-problems"):
         plan.tile = 13
     assert plan.tile == 14
+
+
+@pytest.mark.parametrize("trial", range(4))
+def test_batched_random_lists(trial):
+    """Seeded random lists (kinds, ring sizes, limb split, tile, radix, grid,
+    combine placement), exact against the NTT products."""
+    import torch
+
+    import gpufft_amd as G
+    rng = np.random.default_rng(0x6770756666740200 + trial)
+    splits = [(5, 2, 5, 2), (4, 3, 4, 3), (4, 3, 5, 2), (6, 2, 3, 3)]
+    cnt = int(rng.integers(3, 10))
+    kinds = [int(k) for k in rng.integers(0, 2, cnt)]
+    rps = [int(rng.integers(6 if k == 0 else 5, 15)) for k in kinds]
+    descs, a, b, exact = _workload(kinds, rps, 0x6770756666740300 + trial)
+    plan = G.BatchedPlan(descs, *splits[trial])
+    da, db = to_dev(a.astype(np.int32)), to_dev(b.astype(np.int32))
+    for tile in (14, 13):
+        plan.tile = tile
+        plan.set_tuning(int(rng.integers(3, 6)), int(rng.choice([32, 256, 512])))
+        plan.combine_in_kernel = bool(rng.integers(0, 2))
+        out = torch.full((len(exact),), -5, dtype=torch.int64, device="cuda")
+        err = torch.zeros(len(descs), dtype=torch.float64, device="cuda")
+        plan.execute(da, db, out, err)
+        assert np.all(to_host(err) < 0.5)
+        assert np.array_equal(to_host(out), exact), (tile, kinds, rps)
