@@ -210,3 +210,38 @@ def test_batched_random_lists(trial):
         plan.execute(da, db, out, err)
         assert np.all(to_host(err) < 0.5)
         assert np.array_equal(to_host(out), exact), (tile, kinds, rps)
+
+
+def test_batched_autotune_db(tmp_path):
+    """tune_batched times radix x grid x combine placement x tile, stores the
+    winner in the tuning database, and a second plan gets it applied without
+    timing; the tuned plan stays exact."""
+    import torch
+
+    import gpufft_amd as G
+    from gpufft_amd import autotune as A
+    kinds, rps = [1, 0, 1, 0], [16, 15, 11, 13]
+    descs, a, b, exact = _workload(kinds, rps, 0x67707566667402c4)
+    da, db = to_dev(a.astype(np.int32)), to_dev(b.astype(np.int32))
+    out = torch.zeros(len(exact), dtype=torch.int64, device="cuda")
+    s = torch.cuda.Stream()
+    dbf = str(tmp_path / "tune.json")
+    plan = G.BatchedPlan(descs, **LIMBS)
+    with torch.cuda.stream(s):
+        cand, timings = A.tune_batched(plan, lambda: plan.execute(da, db, out, stream=s), s,
+                                       key="test_batched_db", reps=1, db_file=dbf)
+    s.synchronize()
+    assert len(cand) == 4 and len(timings) == 24
+    assert {c[3] for c in timings} == {13, 14}
+    plan2 = G.BatchedPlan(descs, **LIMBS)
+    with torch.cuda.stream(s):
+        cand2, timings2 = A.tune_batched(plan2, lambda: plan2.execute(da, db, out, stream=s), s,
+                                         key="test_batched_db", reps=1, db_file=dbf)
+        out.zero_()
+        plan2.execute(da, db, out, stream=s)
+    s.synchronize()
+    assert tuple(cand2) == tuple(cand) and timings2 == {}
+    info = plan2.info()
+    assert (info["radix_bits"], info["grid"]) == (cand[0], cand[1])
+    assert plan2.combine_in_kernel == bool(cand[2]) and plan2.tile == cand[3]
+    assert np.array_equal(to_host(out), exact)
