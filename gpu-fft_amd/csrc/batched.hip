@@ -985,12 +985,21 @@ int build_layout(gpufft_batched_plan_s* p, int tile, hipStream_t st)
 int alloc_stash(gpufft_batched_plan_s* p, int grid)
 {
     if (p->stash_slots == 0 || grid <= p->stash_grid) return GPUFFT_OK;
-    if (p->stash) (void)hipFree(p->stash);
-    p->stash = nullptr;
-    p->stash_grid = 0;
-    hipError_t e = hipMalloc((void**)&p->stash,
+    // the new stash first: on failure the plan keeps its old stash (and any
+    // captured graph that points to it) unchanged and usable
+    cplx<float>* fresh = nullptr;
+    hipError_t e = hipMalloc((void**)&fresh,
                              sizeof(cplx<float>) * ((size_t)grid * p->stash_slots << SMAX));
-    if (e != hipSuccess) return fail_hip(e, "gpufft_batched_plan: stash");
+    if (e != hipSuccess) {
+        (void)hipGetLastError();  // the failure is reported here, not by the next launch
+        return fail_hip(e, "gpufft_batched_plan: stash");
+    }
+    if (p->graph) {  // captured with the old stash's address
+        (void)hipGraphExecDestroy(p->graph);
+        p->graph = nullptr;
+    }
+    if (p->stash) (void)hipFree(p->stash);
+    p->stash = fresh;
     p->stash_grid = grid;
     return GPUFFT_OK;
 }

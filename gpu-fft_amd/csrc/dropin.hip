@@ -85,26 +85,6 @@ int r2r(void* real, void* temp, const void* table, const gpufft_config* cfg, int
         io.dst_stride = L;
         return run_transform<T>(MODE_STD_FWD, n, batch, nega, (const cplx<T>*)table, io, st, who);
     }
-    if (!nega && n >= 3) {
-        // X^N-1: half-length inverse (LD_RR_PACK), an (N/2)-point transform
-        // whose complex outputs are the real pairs (c_2m, c_2m+1); the real
-        // buffer holds the intermediate passes.  The real output of the
-        // reference is Re(mod_inverse * D_j) with D_j real for the spectrum
-        // of real data, i.e. Re(mod_inverse) * D_j.
-        const long long M = L / 2;
-        io.lk = LD_RR_PACK;
-        io.src = t;
-        io.src2 = mult ? t + (long long)batch * L : nullptr;
-        io.ltab = (const cplx<T>*)table;
-        io.src_stride = L;
-        io.sk = ST_CPLX;
-        io.dst = real;
-        io.dst_stride = M;
-        io.work = (cplx<T>*)real;
-        io.do_scale = true;
-        io.scale = mk<T>((T)cfg->mod_inverse_re, (T)0);
-        return run_transform<T>(MODE_STD_INV, n - 1, batch, 0, (const cplx<T>*)table, io, st, who);
-    }
     io.lk = mult ? LD_MUL : LD_CPLX;
     io.src = t;
     io.src2 = t + (long long)batch * L;
@@ -115,6 +95,47 @@ int r2r(void* real, void* temp, const void* table, const gpufft_config* cfg, int
     io.do_scale = true;
     io.scale = mk<T>((T)cfg->mod_inverse_re, (T)cfg->mod_inverse_im);
     return run_transform<T>(MODE_STD_INV, n, batch, nega, (const cplx<T>*)table, io, st, who);
+}
+
+// X^N-1 R_R inverse by half-length packing, for spectra of real data only.
+// For k = bitrev(r), C_k and C_{k+M} (M = N/2) sit at temp[2r], temp[2r+1];
+// when temp is the spectrum of real data (Hermitian: C_{N-k} = conj(C_k)),
+// the N-point inverse D is real and D_2m + i D_2m+1 is the M-point inverse
+// of Z_k = (C_k + C_{k+M}) + i w^-k (C_k - C_{k+M}) (LD_RR_PACK), so one
+// transform of half the length gives the real output Re(mod_inverse) D_j,
+// which equals the reference's Re(mod_inverse * D_j) (fft.cu:499-506) for
+// such temp.  For any other temp (or a complex mod_inverse) the result
+// differs from the reference's; gpufft_fft_r2r_* is the general form.
+// X^N+1 and n < 3 run the general inverse.
+template <typename T>
+int r2r_inverse_hermitian(void* real, void* temp, const void* table, const gpufft_config* cfg,
+                          int batch, int mult, const char* who)
+{
+    if (int s = check_cfg(cfg, GPUFFT_MIN_N_POWER, GPUFFT_MAX_N_POWER, who)) return s;
+    if (cfg->fft_type != GPUFFT_INVERSE)
+        return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": fft_type must be INVERSE");
+    const int n = cfg->n_power;
+    const int nega = cfg->reduction_poly == GPUFFT_X_N_PLUS;
+    if (nega || n < 3) return r2r<T>(real, temp, table, cfg, batch, mult, who);
+    if (int s = check_batch(batch, who)) return s;
+    if (!real || !temp || !table) return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": NULL buffer");
+    const long long L = 1LL << n;
+    const long long M = L / 2;
+    cplx<T>* t = (cplx<T>*)temp;
+    TransformIO<T> io{};
+    io.lk = LD_RR_PACK;
+    io.src = t;
+    io.src2 = mult ? t + (long long)batch * L : nullptr;
+    io.ltab = (const cplx<T>*)table;
+    io.src_stride = L;
+    io.sk = ST_CPLX;
+    io.dst = real;
+    io.dst_stride = M;
+    io.work = (cplx<T>*)real;  // the real buffer holds the intermediate passes
+    io.do_scale = true;
+    io.scale = mk<T>((T)cfg->mod_inverse_re, (T)0);
+    return run_transform<T>(MODE_STD_INV, n - 1, batch, 0, (const cplx<T>*)table, io,
+                            (hipStream_t)cfg->stream, who);
 }
 
 template <typename T>
@@ -351,6 +372,18 @@ int gpufft_fft_r2r_f64(void* real, void* temp, const void* t, const gpufft_confi
                        int mult)
 {
     return r2r<double>(real, temp, t, cfg, batch, mult, "gpufft_fft_r2r_f64");
+}
+int gpufft_fft_r2r_inverse_hermitian_f32(void* real, void* temp, const void* t,
+                                         const gpufft_config* cfg, int batch, int mult)
+{
+    return r2r_inverse_hermitian<float>(real, temp, t, cfg, batch, mult,
+                                        "gpufft_fft_r2r_inverse_hermitian_f32");
+}
+int gpufft_fft_r2r_inverse_hermitian_f64(void* real, void* temp, const void* t,
+                                         const gpufft_config* cfg, int batch, int mult)
+{
+    return r2r_inverse_hermitian<double>(real, temp, t, cfg, batch, mult,
+                                         "gpufft_fft_r2r_inverse_hermitian_f64");
 }
 int gpufft_ffnt_f32(void* inout, void* temp, const void* tw, const void* t, const gpufft_config* cfg,
                     int batch, int mult)

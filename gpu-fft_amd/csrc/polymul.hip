@@ -214,6 +214,47 @@ int check_plan(const gpufft_polymul_plan_s* pl, int precision, int batch, const 
     return GPUFFT_OK;
 }
 
+// FP64 products of a validated plan (or of the no-allocation entry point's
+// stack plan), in launches of at most max_batch products.
+int run_f64(const gpufft_polymul_plan_s* plan, const double* a, const double* b, double* out,
+            int batch, int phases, hipStream_t st, const char* who)
+{
+    const long long N = 2LL << plan->n;
+    for (int b0 = 0; b0 < batch; b0 += plan->max_batch) {
+        const int nb = std::min(plan->max_batch, batch - b0);
+        const double* pa = a + b0 * N;
+        const double* pb = b + b0 * N;
+        double* po = out + b0 * N;
+        // in-kernel combine: one launch does both phases (PHASE_TRANSFORM
+        // runs it, PHASE_COMBINE alone does nothing)
+        const int cmb = plan->cmb && plan->SB > 0;
+        if (phases & GPUFFT_PHASE_TRANSFORM)
+            if (int s = launch_products<double, PL_FOLD>(plan, pa, pb, po, nb, cmb, st, who)) return s;
+        if (plan->SB > 0 && !cmb && (phases & GPUFFT_PHASE_COMBINE)) {
+            // top SB inverse stages + scale + untwist + unfold
+            F64CombineArgs ca{};
+            ca.n = plan->n;
+            ca.S = plan->S;
+            ca.batch = nb;
+            ca.scratch = (const cplx<double>*)plan->scratch;
+            ca.itab = (const cplx<double>*)plan->iroot;
+            ca.untwist = (const cplx<double>*)plan->untwist;
+            ca.scale = 1.0 / (double)(1LL << plan->n);
+            ca.out = po;
+            ca.out_stride = N;
+            const unsigned blocks = (unsigned)((((long long)nb << (plan->S - 1)) + 255) / 256);
+            switch (plan->SB) {
+                case 1: hipLaunchKernelGGL(f64_combine_kernel<1>, dim3(blocks), dim3(256), 0, st, ca); break;
+                case 2: hipLaunchKernelGGL(f64_combine_kernel<2>, dim3(blocks), dim3(256), 0, st, ca); break;
+                case 3: hipLaunchKernelGGL(f64_combine_kernel<3>, dim3(blocks), dim3(256), 0, st, ca); break;
+                default: return fail(GPUFFT_ERR_UNSUPPORTED, std::string(who) + ": split too deep");
+            }
+            GFA_CHECK_LAUNCH(who);
+        }
+    }
+    return ok();
+}
+
 }  // namespace
 
 extern "C" {
@@ -315,41 +356,7 @@ int gpufft_polymul_f64_phases(gpufft_polymul_plan plan, const double* a, const d
     const char* who = "gpufft_polymul_f64";
     if (int s = check_plan(plan, GPUFFT_PREC_F64, batch, who)) return s;
     if (!a || !b || !out) return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": NULL buffer");
-    hipStream_t st = (hipStream_t)stream;
-    const long long N = 2LL << plan->n;
-    for (int b0 = 0; b0 < batch; b0 += plan->max_batch) {
-        const int nb = std::min(plan->max_batch, batch - b0);
-        const double* pa = a + b0 * N;
-        const double* pb = b + b0 * N;
-        double* po = out + b0 * N;
-        // in-kernel combine: one launch does both phases (PHASE_TRANSFORM
-        // runs it, PHASE_COMBINE alone does nothing)
-        const int cmb = plan->cmb && plan->SB > 0;
-        if (phases & GPUFFT_PHASE_TRANSFORM)
-            if (int s = launch_products<double, PL_FOLD>(plan, pa, pb, po, nb, cmb, st, who)) return s;
-        if (plan->SB > 0 && !cmb && (phases & GPUFFT_PHASE_COMBINE)) {
-            // top SB inverse stages + scale + untwist + unfold
-            F64CombineArgs ca{};
-            ca.n = plan->n;
-            ca.S = plan->S;
-            ca.batch = nb;
-            ca.scratch = (const cplx<double>*)plan->scratch;
-            ca.itab = (const cplx<double>*)plan->iroot;
-            ca.untwist = (const cplx<double>*)plan->untwist;
-            ca.scale = 1.0 / (double)(1LL << plan->n);
-            ca.out = po;
-            ca.out_stride = N;
-            const unsigned blocks = (unsigned)((((long long)nb << (plan->S - 1)) + 255) / 256);
-            switch (plan->SB) {
-                case 1: hipLaunchKernelGGL(f64_combine_kernel<1>, dim3(blocks), dim3(256), 0, st, ca); break;
-                case 2: hipLaunchKernelGGL(f64_combine_kernel<2>, dim3(blocks), dim3(256), 0, st, ca); break;
-                case 3: hipLaunchKernelGGL(f64_combine_kernel<3>, dim3(blocks), dim3(256), 0, st, ca); break;
-                default: return fail(GPUFFT_ERR_UNSUPPORTED, std::string(who) + ": split too deep");
-            }
-            GFA_CHECK_LAUNCH(who);
-        }
-    }
-    return ok();
+    return run_f64(plan, a, b, out, batch, phases, (hipStream_t)stream, who);
 }
 
 int gpufft_polymul_i32(gpufft_polymul_plan plan, const int32_t* a, const int32_t* b, int64_t* out,
@@ -379,7 +386,9 @@ int gpufft_polymul_i32_phases(gpufft_polymul_plan plan, const int32_t* a, const 
                                                              nb, 0, st, who))
                 return s;
         if (!(phases & GPUFFT_PHASE_COMBINE)) continue;
-        const int nthreads_total = nb << plan->S;
+        const long long nthreads_total = (long long)nb << plan->S;
+        if ((nthreads_total + 255) / 256 > 0x7fffffffLL)
+            return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": batch too large");
         const unsigned blocks = (unsigned)((nthreads_total + 255) / 256);
 #define GFA_COMB(BB)                                                                       \
     case BB: {                                                                             \

@@ -14,10 +14,9 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-# GPUFFT_AMD_LIB may name another build of the same library (e.g. one compiled
-# with different tuning macros); by default the in-tree build is loaded.
-LIB_PATH = os.environ.get("GPUFFT_AMD_LIB") or os.path.join(os.path.dirname(_HERE), "lib",
-                                                            "libgpufft_amd.so")
+# always the in-tree build: tests and benchmarks must run the library built
+# from this source tree
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libgpufft_amd.so")
 
 FORWARD, INVERSE = 0, 1
 X_N_PLUS, X_N_MINUS = 0, 1
@@ -68,6 +67,8 @@ _SIGS = {
     "gpufft_fft_c2c_f64": [_P, _P, _CP, _I, _I],
     "gpufft_fft_r2r_f32": [_P, _P, _P, _CP, _I, _I],
     "gpufft_fft_r2r_f64": [_P, _P, _P, _CP, _I, _I],
+    "gpufft_fft_r2r_inverse_hermitian_f32": [_P, _P, _P, _CP, _I, _I],
+    "gpufft_fft_r2r_inverse_hermitian_f64": [_P, _P, _P, _CP, _I, _I],
     "gpufft_ffnt_f32": [_P, _P, _P, _P, _CP, _I, _I],
     "gpufft_ffnt_f64": [_P, _P, _P, _P, _CP, _I, _I],
     "gpufft_special_fft_f32": [_P, _P, _CP, _I],
@@ -187,6 +188,16 @@ def GPU_FFT(inout, root_table, cfg, batch_size, multiplication=False):
 def GPU_FFT_R(real_inout, temp, root_table, cfg, batch_size, multiplication=False):
     """Real-to-complex / complex-to-real GPU_FFT (reference fft.cuh:126-132)."""
     fn = getattr(_lib, "gpufft_fft_r2r_" + _prec(temp))
+    check(fn(_ptr(real_inout), _ptr(temp), _ptr(root_table), ctypes.byref(cfg), batch_size,
+             int(multiplication)))
+
+
+def GPU_FFT_R_inverse_hermitian(real_inout, temp, root_table, cfg, batch_size,
+                                multiplication=False):
+    """R_R inverse for spectra of real data only (half-length packed form for
+    X^N-1); equals GPU_FFT_R's INVERSE when temp is Hermitian and mod_inverse
+    is real."""
+    fn = getattr(_lib, "gpufft_fft_r2r_inverse_hermitian_" + _prec(temp))
     check(fn(_ptr(real_inout), _ptr(temp), _ptr(root_table), ctypes.byref(cfg), batch_size,
              int(multiplication)))
 

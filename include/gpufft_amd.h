@@ -82,6 +82,25 @@ int gpufft_fft_r2r_f32(void* real_inout, void* temp, const void* root_table,
 int gpufft_fft_r2r_f64(void* real_inout, void* temp, const void* root_table,
                        const gpufft_config* cfg, int batch, int multiplication);
 
+/* The R_R INVERSE for the common case where temp holds spectra of real data
+ * (as the R_R FORWARD writes them; Hermitian: C[N-k] = conj(C[k])), so the
+ * product of two such spectra is one too.  For X^N-1 (n_power >= 3) it runs
+ * a half-length transform of the packed spectrum, half the flops and less
+ * HBM traffic than the general inverse, and writes
+ * real[b][i] = Re(mod_inverse) * D[i]; under the precondition that equals
+ * the general result Re(mod_inverse * D[i]) (D is then real).  For any other
+ * temp, or a mod_inverse with an imaginary part, the result differs from the
+ * reference's: use gpufft_fft_r2r_* then.  The real buffer also holds the
+ * intermediate passes; temp is read only.  X^N+1 and n_power < 3 run the
+ * general inverse.  fft_type must be GPUFFT_INVERSE.
+ * No counterpart in the reference (whose GPU_FFT R_R inverse, fft.cu:600-725,
+ * always runs the full-length complex inverse); for callers that feed the
+ * R_R inverse only with R_R forward spectra. */
+int gpufft_fft_r2r_inverse_hermitian_f32(void* real_inout, void* temp, const void* root_table,
+                                         const gpufft_config* cfg, int batch, int multiplication);
+int gpufft_fft_r2r_inverse_hermitian_f64(void* real_inout, void* temp, const void* root_table,
+                                         const gpufft_config* cfg, int batch, int multiplication);
+
 /* FFNT negacyclic transform (ring size N = 2^n_power, length-N/2 complex FFT).
  * FORWARD: real inout[batch][N] -> fold z_j = f_j + i f_{j+N/2}, times
  *   twist_table[j], forward FFT -> temp[batch][N/2] (bit-reversed).

@@ -245,3 +245,94 @@ def test_batched_autotune_db(tmp_path):
     assert (info["radix_bits"], info["grid"]) == (cand[0], cand[1])
     assert plan2.combine_in_kernel == bool(cand[2]) and plan2.tile == cand[3]
     assert np.array_equal(to_host(out), exact)
+
+
+def test_batched_failed_retune_keeps_plan_usable():
+    """A grid whose stash cannot be allocated is refused; the plan keeps its
+    old stash, tuning and captured graph, and still gives exact products."""
+    import torch
+
+    import gpufft_amd as G
+    kinds, rps = [0, 1, 0], [15, 12, 11]
+    descs, a, b, exact = _workload(kinds, rps, 0x6770756666740045)
+    plan = G.BatchedPlan(descs, **LIMBS)
+    da, db = to_dev(a.astype(np.int32)), to_dev(b.astype(np.int32))
+    out = torch.zeros(len(exact), dtype=torch.int64, device="cuda")
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        plan.graph_launch(da, db, out, stream=s)
+    s.synchronize()
+    assert np.array_equal(to_host(out), exact)
+    before = plan.info()
+    with pytest.raises(G.HipError):
+        plan.set_tuning(4, 1 << 28)  # a stash of about 10^14 bytes
+    assert plan.info() == before
+    out.zero_()
+    with torch.cuda.stream(s):
+        plan.graph_launch(da, db, out, stream=s)
+    s.synchronize()
+    assert np.array_equal(to_host(out), exact)
+    out.zero_()
+    plan.execute(da, db, out)
+    assert np.array_equal(to_host(out), exact)
+
+
+def _exact_products(descs, a, b, threads=16):
+    """Exact products of every descriptor by the oracle's NTT, in a thread
+    pool (each oracle call releases the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import gpufft_amd as G
+
+    def one(d):
+        N = 1 << d.ring_power
+        x = a[d.a_offset:d.a_offset + N].astype(np.int64)
+        y = b[d.b_offset:d.b_offset + N].astype(np.int64)
+        if d.kind == G.PRODUCT_NEGACYCLIC:
+            return O.ntt_negacyclic(x, y)
+        xp = np.zeros(2 * N, np.int64)
+        yp = np.zeros(2 * N, np.int64)
+        xp[:N], yp[:N] = x, y
+        return O.ntt_cyclic(xp, yp)
+
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        return list(ex.map(one, list(descs)))
+
+
+def test_batched_configured_size_4096():
+    """The batched uneven configuration at its full size: 4096 products, ring
+    sizes 2^e with e uniform in {10..17} (seeded), even index linear, odd
+    negacyclic, coefficients uniform in [-2^8, 2^8) (the seeds of bench.py
+    --batched-uneven), one persistent launch replayed as a HIP graph twice,
+    every product bit-exact against the oracle's NTT."""
+    import torch
+
+    import gpufft_amd as G
+    from gpufft_amd import synthetic as S
+    count = 4096
+    e = S.uniform(S.SEED_BASE + 4, 10, 18, count)
+    kinds = [G.PRODUCT_LINEAR if i % 2 == 0 else G.PRODUCT_NEGACYCLIC for i in range(count)]
+    descs, n_in, n_out = G.pack_products(kinds, e)
+    a = S.uniform(S.SEED_BASE + 4 + 0x100, -256, 256, n_in).astype(np.int32)
+    b = S.uniform(S.SEED_BASE + 4 + 0x200, -256, 256, n_in).astype(np.int32)
+    print("batched 4096: exact products on the host", flush=True)
+    exact = _exact_products(descs, a, b)
+    plan = G.BatchedPlan(descs, **LIMBS)
+    info = plan.info()
+    assert info["items"] > count  # the large products are split
+    da, db = to_dev(a), to_dev(b)
+    out = torch.zeros(n_out, dtype=torch.int64, device="cuda")
+    err = torch.zeros(count, dtype=torch.float64, device="cuda")
+    s = torch.cuda.Stream()
+    for rep in range(2):
+        print("batched 4096: graph replay", rep, flush=True)
+        with torch.cuda.stream(s):
+            out.fill_(-1)
+            plan.graph_launch(da, db, out, err, stream=s)
+        s.synchronize()
+        got = to_host(out)
+        e_host = to_host(err)
+        assert np.all(e_host < 0.5), float(e_host.max())
+        bad = [i for i, d in enumerate(descs)
+               if not np.array_equal(got[d.out_offset:d.out_offset + len(exact[i])], exact[i])]
+        assert not bad, (rep, len(bad), bad[:8])

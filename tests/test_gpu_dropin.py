@@ -243,55 +243,105 @@ def test_c2c_negacyclic_product_exact(n_power):
     assert np.array_equal(got, O.ntt_negacyclic(a, b))
 
 
-# ---- the top of the drop-in range (n_power 24: 2^24 points, four passes) ----
+# ---- the top of the drop-in range (n_power 24: 2^24 points, four passes;
+# float splits into three passes from n_power 23) ----
+@pytest.mark.parametrize("prec,n_power", [("f64", 24), ("f32", 23), ("f32", 24)])
 @pytest.mark.parametrize("kind", ["c2c", "c2c_nega", "r2r", "ffnt", "special"])
-def test_dropin_max_size(kind):
-    """Every drop-in entry point at n_power = 24 (reference maximum,
-    fft.cu:522-525), batch 1, FP64, against the oracle."""
+def test_dropin_max_size(kind, prec, n_power):
+    """Every drop-in entry point at the top of the reference's range
+    (n_power 24, fft.cu:522-525), batch 1, FP64 and FP32, against the
+    oracle."""
     G = _g()
     import torch
-    n_power = 24
-    rng = np.random.default_rng(7700 + len(kind))
+    rng = np.random.default_rng(7700 + len(kind) + n_power)
     L = 1 << n_power
+    cdt = torch.complex128 if prec == "f64" else torch.complex64
+    tol = TOL[prec]
     if kind in ("c2c", "c2c_nega"):
         nega = kind == "c2c_nega"
-        tabs = _tables(n_power, nega, "f64")
-        x = rng_complex(rng, L, "f64")
+        tabs = _tables(n_power, nega, prec)
+        x = rng_complex(rng, L, prec)
         d = to_dev(x)
         rp = G.X_N_PLUS if nega else G.X_N_MINUS
         G.GPU_FFT(d, to_dev(tabs["rev"]), G.config(n_power, G.FORWARD, rp), 1)
-        ref = O.c2c_forward(x, n_power, 1, tabs["rev"], nega)
-        assert rel_err(to_host(d), ref) < TOL["f64"]
+        ref = O.c2c_forward(x, n_power, 1, tabs["rev"], nega, prec)
+        assert rel_err(to_host(d), ref) < tol
         G.GPU_FFT(d, to_dev(tabs["inv_rev"]), G.config(n_power, G.INVERSE, rp, 1.0 / L), 1)
-        assert rel_err(to_host(d), x) < 100 * TOL["f64"]
+        assert rel_err(to_host(d), x) < 100 * tol
     elif kind == "r2r":
-        tabs = _tables(n_power, 0, "f64")
-        real = rng.standard_normal(L)
+        tabs = _tables(n_power, 0, prec)
+        real = rng.standard_normal(L).astype(rdtype(prec))
         dr = to_dev(real)
-        dt = torch.zeros(L, dtype=torch.complex128, device="cuda")
+        dt = torch.zeros(L, dtype=cdt, device="cuda")
         G.GPU_FFT_R(dr, dt, to_dev(tabs["rev"]), G.config(n_power, G.FORWARD), 1)
-        ref = O.r2r_forward(real, n_power, 1, tabs["rev"], 0)
-        assert rel_err(to_host(dt), ref) < TOL["f64"]
+        ref = O.r2r_forward(real, n_power, 1, tabs["rev"], 0, prec)
+        assert rel_err(to_host(dt), ref) < tol
         G.GPU_FFT_R(dr, dt, to_dev(tabs["inv_rev"]), G.config(n_power, G.INVERSE, mod_inverse=1.0 / L), 1)
-        assert rel_err(to_host(dr), real) < 100 * TOL["f64"]
+        assert rel_err(to_host(dr), real) < 100 * tol
     elif kind == "ffnt":
         N, Lh = L, L // 2
-        t = O.ffnt_tables(N)
-        real = rng.standard_normal(N)
+        t = O.ffnt_tables(N, prec)
+        real = rng.standard_normal(N).astype(rdtype(prec))
         dr = to_dev(real)
-        dt = torch.zeros(Lh, dtype=torch.complex128, device="cuda")
+        dt = torch.zeros(Lh, dtype=cdt, device="cuda")
         G.GPU_FFNT(dr, dt, to_dev(t["twist"]), to_dev(t["rev_root"]), G.config(n_power, G.FORWARD), 1)
-        ref = O.ffnt_forward(real, n_power, 1, t["twist"], t["rev_root"])
-        assert rel_err(to_host(dt), ref) < TOL["f64"]
+        ref = O.ffnt_forward(real, n_power, 1, t["twist"], t["rev_root"], prec)
+        assert rel_err(to_host(dt), ref) < tol
         G.GPU_FFNT(dr, dt, to_dev(t["untwist"]), to_dev(t["inv_rev_root"]),
                    G.config(n_power, G.INVERSE, mod_inverse=1.0 / Lh), 1)
-        assert rel_err(to_host(dr), real) < 100 * TOL["f64"]
+        assert rel_err(to_host(dr), real) < 100 * tol
     else:
-        T, Ti = O.special_table(L), O.special_table(L, True)
-        z = rng_complex(rng, L, "f64")
+        T, Ti = O.special_table(L, False, prec), O.special_table(L, True, prec)
+        z = rng_complex(rng, L, prec)
         d = to_dev(z)
         G.GPU_Special_FFT(d, to_dev(T), G.config(n_power, G.FORWARD), 1)
-        ref = O.special_forward(z, n_power, 1, T)
-        assert rel_err(to_host(d), ref) < 4 * TOL["f64"]
+        ref = O.special_forward(z, n_power, 1, T, prec)
+        assert rel_err(to_host(d), ref) < 4 * tol
         G.GPU_Special_FFT(d, to_dev(Ti), G.config(n_power, G.INVERSE, mod_inverse=1.0 / L), 1)
-        assert rel_err(to_host(d), z) < 100 * TOL["f64"]
+        assert rel_err(to_host(d), z) < 100 * tol
+
+
+# ---- R_R inverse on arbitrary temp (reference fft.cu:499-506: the output is
+# Re(mod_inverse * D) for any temp, D the complex GS inverse) ----
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("mult", [0, 1])
+@pytest.mark.parametrize("n_power", [3, 12, 16])
+def test_r2r_inverse_non_hermitian(prec, mult, n_power):
+    """temp holds arbitrary complex values (not the spectrum of real data)
+    and mod_inverse is complex: the drop-in R_R inverse must still give the
+    reference's Re(mod_inverse * D_j)."""
+    G = _g()
+    rng = np.random.default_rng(8100 + n_power * 4 + mult * 2 + (prec == "f32"))
+    batch, L = 2, 1 << n_power
+    tabs = _tables(n_power, 0, prec)
+    temp = rng_complex(rng, 2 * batch * L, prec)
+    mi = complex(1.0 / L, -0.61 / L)
+    nb = batch if mult else 2 * batch
+    dr = to_dev(np.zeros(2 * batch * L, rdtype(prec)))
+    dt = to_dev(temp)
+    G.GPU_FFT_R(dr, dt, to_dev(tabs["inv_rev"]), G.config(n_power, G.INVERSE, G.X_N_MINUS, mi),
+                nb, bool(mult))
+    ref = O.r2r_inverse(temp, n_power, nb, tabs["inv_rev"], 0, mult, mi, prec)
+    assert rel_err(to_host(dr)[: nb * L], ref) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("n_power", [2, 3, 12, 16, 17])
+def test_r2r_inverse_hermitian_entry(prec, n_power):
+    """The opt-in packed inverse equals the general one on spectra of real
+    data (its precondition), with and without the product."""
+    G = _g()
+    import torch
+    rng = np.random.default_rng(8300 + n_power * 2 + (prec == "f32"))
+    batch, L = 2, 1 << n_power
+    tabs = _tables(n_power, 0, prec)
+    real = rng.standard_normal(2 * batch * L).astype(rdtype(prec))
+    spec = O.r2r_forward(real, n_power, 2 * batch, tabs["rev"], 0, prec)
+    for mult, nb in ((1, batch), (0, 2 * batch)):
+        dr = torch.zeros(2 * batch * L, dtype=torch.float64 if prec == "f64" else torch.float32,
+                         device="cuda")
+        G.GPU_FFT_R_inverse_hermitian(dr, to_dev(spec), to_dev(tabs["inv_rev"]),
+                                      G.config(n_power, G.INVERSE, G.X_N_MINUS, 1.0 / L), nb,
+                                      bool(mult))
+        ref = O.r2r_inverse(spec, n_power, nb, tabs["inv_rev"], 0, mult, complex(1.0 / L), prec)
+        assert rel_err(to_host(dr)[: nb * L], ref) < TOL[prec]

@@ -118,6 +118,28 @@ def test_r2r_matches_c2c_on_real_data(n_power):
     assert np.array_equal(r, c.real)
 
 
+@pytest.mark.parametrize("n_power", [1, 3, 8])
+@pytest.mark.parametrize("mult", [0, 1])
+def test_r2r_inverse_general_temp(n_power, mult):
+    """R_R inverse on arbitrary (non-Hermitian) temp with a complex
+    mod_inverse: real_j = Re(mi * D_j), D_j = sum_k C_k w^{-jk} with
+    C_k = temp[bitrev(k)] (times the second operand with mult); a dropped
+    Im(mi) or Im(D) term fails here."""
+    L = 1 << n_power
+    t = O.fft_tables(max(1, L // 2))
+    rng = np.random.default_rng(40 + n_power + 7 * mult)
+    temp = rng.standard_normal(2 * L) + 1j * rng.standard_normal(2 * L)
+    mi = complex(0.3, -0.7)
+    nb = 1 if mult else 2
+    got = O.r2r_inverse(temp, n_power, nb, t["inv_rev"], 0, mult, mi)
+    for b in range(nb):
+        C = temp[b * L:(b + 1) * L] * (temp[L:2 * L] if mult else 1)
+        D = np.fft.fft(C[br(n_power)])
+        ref = (mi * D).real
+        assert np.max(np.abs(got[b * L:(b + 1) * L] - ref)) < 1e-12 * L
+        assert np.max(np.abs(ref - mi.real * D.real)) > 1e-3  # the Im parts matter here
+
+
 @pytest.mark.parametrize("n_power", [2, 4, 9, 13])
 def test_ffnt_is_negacyclic_product(n_power):
     N = 1 << n_power
