@@ -31,6 +31,17 @@ int check_batch(int batch, const char* who)
     return GPUFFT_OK;
 }
 
+// Out-of-place transform (source and destination buffers differ): one
+// launch where the one-launch path covers the size, else the pass chain.
+template <typename T>
+int transform_oop(int mode, int n, int batch, int nega, const cplx<T>* tab, const TransformIO<T>& io,
+                  hipStream_t st, const char* who)
+{
+    int s = GPUFFT_OK;
+    if (run_oneshot<T>(mode, n, batch, nega, tab, io, st, who, &s)) return s;
+    return run_transform<T>(mode, n, batch, nega, tab, io, st, who);
+}
+
 template <typename T>
 int c2c(void* inout, const void* table, const gpufft_config* cfg, int batch, int mult,
         const char* who)
@@ -83,7 +94,7 @@ int r2r(void* real, void* temp, const void* table, const gpufft_config* cfg, int
         io.sk = ST_CPLX;
         io.dst = t;
         io.dst_stride = L;
-        return run_transform<T>(MODE_STD_FWD, n, batch, nega, (const cplx<T>*)table, io, st, who);
+        return transform_oop<T>(MODE_STD_FWD, n, batch, nega, (const cplx<T>*)table, io, st, who);
     }
     io.lk = mult ? LD_MUL : LD_CPLX;
     io.src = t;
@@ -94,7 +105,7 @@ int r2r(void* real, void* temp, const void* table, const gpufft_config* cfg, int
     io.dst_stride = L;
     io.do_scale = true;
     io.scale = mk<T>((T)cfg->mod_inverse_re, (T)cfg->mod_inverse_im);
-    return run_transform<T>(MODE_STD_INV, n, batch, nega, (const cplx<T>*)table, io, st, who);
+    return transform_oop<T>(MODE_STD_INV, n, batch, nega, (const cplx<T>*)table, io, st, who);
 }
 
 // X^N-1 R_R inverse by half-length packing, for spectra of real data only.
@@ -160,7 +171,7 @@ int ffnt(void* inout, void* temp, const void* twist, const void* table, const gp
         io.sk = ST_CPLX;
         io.dst = t;
         io.dst_stride = L;
-        return run_transform<T>(MODE_STD_FWD, n, batch, 0, (const cplx<T>*)table, io, st, who);
+        return transform_oop<T>(MODE_STD_FWD, n, batch, 0, (const cplx<T>*)table, io, st, who);
     }
     io.lk = mult ? LD_MUL : LD_CPLX;
     io.src = t;
@@ -172,7 +183,7 @@ int ffnt(void* inout, void* temp, const void* twist, const void* table, const gp
     io.dst_stride = 2 * L;
     io.do_scale = true;
     io.scale = mk<T>((T)cfg->mod_inverse_re, (T)cfg->mod_inverse_im);
-    return run_transform<T>(MODE_STD_INV, n, batch, 0, (const cplx<T>*)table, io, st, who);
+    return transform_oop<T>(MODE_STD_INV, n, batch, 0, (const cplx<T>*)table, io, st, who);
 }
 
 template <typename T>

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "oneshot.hpp"
 #include "status.hpp"
 
 namespace gfa {
@@ -177,6 +178,86 @@ int run_transform(int mode, int n, int batch, int nega, const cplx<T>* tab,
         GFA_CHECK_LAUNCH(who);
     }
     return ok();
+}
+
+// One-launch path of the out-of-place calls (R_R, FFNT): a transform of
+// 2^n points, n = S + SB with S the LDS tile (13 double, 14 float) and
+// SB = 1..3, is one launch of Q = 2^SB workgroups per transform.
+template <typename T> struct OneCfg;
+template <> struct OneCfg<double> {
+    static constexpr int E = 4;  // 16 points per thread, tile 2^13
+};
+template <> struct OneCfg<float> {
+    static constexpr int E = 5;  // 32 points per thread, tile 2^14
+};
+
+template <typename T, int SB> PassFn<T> select_oneshot_sb(int mode, int lk, int sk)
+{
+    constexpr int E = OneCfg<T>::E;
+#define GFA_ONE_F(L, S) \
+    if (mode == MODE_STD_FWD && lk == L && sk == S) return &oneshot_forward_kernel<T, E, SB, L, S>;
+#define GFA_ONE_I(L, S) \
+    if (mode == MODE_STD_INV && lk == L && sk == S) return &oneshot_inverse_kernel<T, E, SB, L, S>;
+    GFA_ONE_F(LD_REAL, ST_CPLX)
+    GFA_ONE_F(LD_FOLD, ST_CPLX)
+    GFA_ONE_I(LD_CPLX, ST_REAL)
+    GFA_ONE_I(LD_MUL, ST_REAL)
+    GFA_ONE_I(LD_CPLX, ST_UNFOLD)
+    GFA_ONE_I(LD_MUL, ST_UNFOLD)
+#undef GFA_ONE_F
+#undef GFA_ONE_I
+    return nullptr;
+}
+
+// Returns true and launches when the one-launch path covers this transform
+// (status in *status); false leaves the pass chain to the caller.  The
+// source must not overlap the destination.
+template <typename T>
+bool run_oneshot(int mode, int n, int batch, int nega, const cplx<T>* tab,
+                 const TransformIO<T>& io, hipStream_t stream, const char* who, int* status)
+{
+    constexpr int S = OneCfg<T>::E + 9;
+    const int SB = n - S;
+    if (SB < 1 || SB > 3) return false;
+    PassFn<T> fn = nullptr;
+    switch (SB) {
+        case 1: fn = select_oneshot_sb<T, 1>(mode, io.lk, io.sk); break;
+        case 2: fn = select_oneshot_sb<T, 2>(mode, io.lk, io.sk); break;
+        case 3: fn = select_oneshot_sb<T, 3>(mode, io.lk, io.sk); break;
+    }
+    if (!fn) return false;
+    PassArgs<T> a{};
+    a.n = n;
+    a.blo = 0;
+    a.K = S;
+    a.logC = 0;
+    a.logBT = 0;
+    a.batch = batch;
+    a.nega = nega;
+    a.tab = tab;
+    a.src = io.src;
+    a.src2 = io.src2;
+    a.ltab = io.ltab;
+    a.src_stride = io.src_stride;
+    a.dst = io.dst;
+    a.stab = io.stab;
+    a.dst_stride = io.dst_stride;
+    a.do_scale = io.do_scale ? 1 : 0;
+    a.scale = io.scale;
+    // the sibling workgroups of a transform re-read its input from L2:
+    // cached loads; every output is written once (the inverse writes every
+    // Q-th output, so its lines fill up from Q workgroups in L2: cached)
+    a.nt_in = 0;
+    a.nt_out = mode == MODE_STD_FWD ? 1 : 0;
+    const long long blocks = (((long long)batch + 7) / 8) * 8 * (1LL << SB);
+    if (blocks > 0x7fffffffLL) {
+        *status = fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": batch too large");
+        return true;
+    }
+    hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(512), 0, stream, a);
+    hipError_t e = hipGetLastError();
+    *status = e != hipSuccess ? fail_hip(e, who) : ok();
+    return true;
 }
 
 }  // namespace gfa
