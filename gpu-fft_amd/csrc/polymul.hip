@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 
 #include "fused.hpp"
@@ -253,6 +254,40 @@ int run_f64(const gpufft_polymul_plan_s* plan, const double* a, const double* b,
         }
     }
     return ok();
+}
+
+// ---- products without allocation ----
+// Static device scratch of gpufft_polymul_f64_noalloc: the stage root
+// constants (64 complex doubles in the first 1 KiB, written by every call)
+// and the sub-problem results of products split over several workgroups.
+// Each device holds its own copy from module load until the process ends;
+// calls that overlap in time on one device share it and race.
+constexpr size_t NOALLOC_BYTES = (size_t)64 << 20;
+constexpr size_t NOALLOC_CONST_BYTES = 1024;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex noalloc_mu;
+void* noalloc_ptr[NOALLOC_MAX_DEVICES];  // device address of `scratch`, per device
+
+// Device address of this device's `scratch`, looked up once per device with
+// the device current (the host-side name is only a stand-in).
+int noalloc_scratch(int dev, void** out, const char* who)
+{
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+        return fail(GPUFFT_ERR_UNSUPPORTED, std::string(who) + ": device index out of range");
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (!noalloc_ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return fail_hip(e != hipSuccess ? e : hipErrorInvalidSymbol, who);
+        }
+        noalloc_ptr[dev] = p;
+    }
+    *out = noalloc_ptr[dev];
+    return GPUFFT_OK;
 }
 
 }  // namespace
@@ -501,6 +536,54 @@ int gpufft_polymul_plan_set_a_center(gpufft_polymul_plan plan, int center_log2)
 int gpufft_polymul_plan_get_a_center(gpufft_polymul_plan plan)
 {
     return plan ? plan->ca : -1;
+}
+
+int gpufft_polymul_f64_noalloc(const double* a, const double* b, double* out, int ring_power,
+                               int batch, const void* twist, const void* untwist,
+                               const void* root_table, const void* inv_root_table, void* stream)
+{
+    const char* who = "gpufft_polymul_f64_noalloc";
+    const int n = ring_power - 1;
+    if (n < 4 + 2 || n > F64_MAX_S + MAX_SB)
+        return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": ring_power must be 7..17");
+    if (batch < 1) return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": batch must be >= 1");
+    if (!a || !b || !out || !twist || !untwist || !root_table || !inv_root_table)
+        return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": NULL buffer");
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail_hip(e, who);
+    void* base = nullptr;
+    if (int s = noalloc_scratch(dev, &base, who)) return s;
+    hipStream_t st = (hipStream_t)stream;
+    gpufft_polymul_plan_s pl{};
+    pl.device = dev;
+    pl.ring_power = ring_power;
+    pl.n = n;
+    pl.precision = GPUFFT_PREC_F64;
+    pl.ka = pl.kb = 1;
+    pl.S = std::min(n, F64_MAX_S);
+    pl.SB = n - pl.S;
+    pl.rb = 4;
+    pl.cmb = 0;
+    pl.root = const_cast<void*>(root_table);
+    pl.iroot = const_cast<void*>(inv_root_table);
+    pl.twist = const_cast<void*>(twist);
+    pl.untwist = const_cast<void*>(untwist);
+    cplx<double>* consts = (cplx<double>*)base;
+    pl.wc = consts;
+    pl.wci = consts + 32;
+    // the sub-problem results of one launch must fit the rest of the array:
+    // the batch runs in launches of at most that many products
+    const size_t per_product = pl.SB > 0 ? (sizeof(cplx<double>) << n) : 0;
+    const size_t room = NOALLOC_BYTES - NOALLOC_CONST_BYTES;
+    pl.max_batch = per_product ? (int)std::min<size_t>(room / per_product, (size_t)batch) : batch;
+    pl.scratch = pl.SB > 0 ? (void*)((unsigned char*)base + NOALLOC_CONST_BYTES) : nullptr;
+    pl.scratch_bytes = pl.SB > 0 ? room : 0;
+    // stage root constants: rewritten by every call (a kernel, so the call
+    // can be captured into a graph), ahead of the product kernels on `st`
+    if (int s = gpufft_generate_table_f64(pl.wc, GPUFFT_TABLE_STAGE_ROOTS, 5, 0, st)) return s;
+    if (int s = gpufft_generate_table_f64(pl.wci, GPUFFT_TABLE_STAGE_ROOTS, 5, 1, st)) return s;
+    return run_f64(&pl, a, b, out, batch, GPUFFT_PHASE_ALL, st, who);
 }
 
 }  // extern "C"

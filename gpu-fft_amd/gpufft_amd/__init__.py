@@ -81,6 +81,7 @@ _SIGS = {
     "gpufft_polymul_f64": [_P, _P, _P, _P, _I, _P],
     "gpufft_polymul_i32": [_P, _P, _P, _P, _I, _P, _P],
     "gpufft_polymul_f64_phases": [_P, _P, _P, _P, _I, _I, _P],
+    "gpufft_polymul_f64_noalloc": [_P, _P, _P, _I, _I, _P, _P, _P, _P, _P],
     "gpufft_polymul_plan_set_radix_bits": [_P, _I],
     "gpufft_batched_plan_create": [_P, _P, _I, _I, _I, _I, _I, _P],
     "gpufft_batched_plan_destroy": [_P],

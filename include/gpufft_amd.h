@@ -233,6 +233,25 @@ int gpufft_polymul_i32_phases(gpufft_polymul_plan plan, const int32_t* a, const 
                               int64_t* out, int batch, double* err_stats, int phases,
                               void* stream);
 
+/* The FP64 product of gpufft_polymul_f64 without a plan and without any
+ * allocation: no counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The caller passes the inputs, the
+ * output and the four FFNT tables of ring N = 2^ring_power (N/2 entries
+ * each, the layouts GPU_FFNT takes: twist, untwist, root and inverse root,
+ * e.g. from gpufft_generate_table_f64); the library uses a static device
+ * array of 64 MiB (per device, held until the process ends) for the stage
+ * constants and the sub-problem results, so the batch runs in launches of
+ * at most 63 MiB / (8 N) products for ring_power >= 15.  ring_power 7..17
+ * (else GPUFFT_ERR_INVALID_ARGUMENT); a, b, out: double [batch][N]; out is
+ * the unrounded product mod X^N+1, equal to gpufft_polymul_f64's.  Calls
+ * that overlap in time on one device share the array and race; calls on one
+ * stream are ordered.  Only kernels are enqueued (no memset, no
+ * allocation), so a call can be captured into a HIP graph. */
+int gpufft_polymul_f64_noalloc(const double* a, const double* b, double* out, int ring_power,
+                               int batch, const void* twist, const void* untwist,
+                               const void* root_table, const void* inv_root_table, void* stream);
+
 /* ---- CKKS encode / decode around the special FFT (no counterpart in the
  * reference, which documents these caller steps around GPU_Special_FFT in
  * src/include/gpufft/fft.cuh:134-180 without implementing them) ----
