@@ -234,9 +234,12 @@ __device__ __forceinline__ void group_stages(cplx<T>* v, const PassArgs<T>& a, c
 // s = RHO-1-i, t < 2^s the group bits above i (the reference tables hold
 // bit-reversed powers, fft_cpu.cu:200-211 and 434-446, so an index t below
 // a multiple of 2^s only adds a root of order 2^(s+1)).  One table load per
-// stage; wc[2^s + t] holds the root constants (s < 5).  The values differ
-// from direct table reads by rounding only.
-template <typename T, int MODE, int RHO>
+// stage; wc[2^s + t] holds the root constants (s < 5), or, with TW, the
+// root table itself does at wc[t]: every reference table (FFT<T>, FFNT<T>,
+// forward or inverse, either reduction) has tab[t] = exp(+-2 pi i bitrev_s(t)
+// / 2^(s+1)) for t < 2^s.  The values differ from direct table reads by
+// rounding only.
+template <typename T, int MODE, int RHO, bool TW = false>
 __device__ __forceinline__ void group_stages_derived(cplx<T>* v, const PassArgs<T>& a,
                                                      const TileGeo& g, int rbase, int col, int q,
                                                      const cplx<T>* __restrict__ wc)
@@ -257,7 +260,7 @@ __device__ __forceinline__ void group_stages_derived(cplx<T>* v, const PassArgs<
                 // inverse chains (Gentleman-Sande) their conjugates
                 w = mode_is_ct(MODE) ? mk<T>(-base.y, base.x) : mk<T>(base.y, -base.x);
             } else if (t > 0) {
-                w = base * wc[(1 << s) + t];
+                w = base * wc[TW ? t : (1 << s) + t];
             }
 #pragma unroll
             for (int lo = 0; lo < (1 << i); lo++) {
@@ -293,7 +296,7 @@ __device__ __forceinline__ void group_of(int tid, int k, int logC, int q, int& r
 // One radix-2^RHO step over tile-row bits [q, q+RHO): every thread takes
 // EPT/2^RHO groups of 2^RHO elements from LDS, runs the RHO stages in
 // registers and writes them back.
-template <typename T, int MODE, int E, int RHO, int NT, bool DRV = false>
+template <typename T, int MODE, int E, int RHO, int NT, bool DRV = false, bool TW = false>
 __device__ __forceinline__ void radix_step(cplx<T>* lds, const PassArgs<T>& a, const TileGeo& g,
                                            int q, int tid, const cplx<T>* wc = nullptr)
 {
@@ -310,7 +313,7 @@ __device__ __forceinline__ void radix_step(cplx<T>* lds, const PassArgs<T>& a, c
 #pragma unroll
         for (int j = 0; j < GS; j++) v[j] = lds[swz(((rbase | (j << q)) << g.logC) | col)];
         if constexpr (DRV)
-            group_stages_derived<T, MODE, RHO>(v, a, g, rbase, col, q, wc);
+            group_stages_derived<T, MODE, RHO, TW>(v, a, g, rbase, col, q, wc);
         else
             group_stages<T, MODE, RHO>(v, a, g, rbase, col, q);
 #pragma unroll
@@ -369,7 +372,8 @@ __device__ __forceinline__ void run_steps(cplx<T>* lds, const PassArgs<T>& a, co
 // Steps [S, SEND) of a pass of K bits with radix-2^RB steps, unrolled at
 // compile time so that every step's bit range is a constant (address and
 // twiddle-index arithmetic then folds).
-template <typename T, int MODE, int E, int NT, int K, int RB, int S, int SEND, bool DRV>
+template <typename T, int MODE, int E, int NT, int K, int RB, int S, int SEND, bool DRV,
+          bool TW = false>
 __device__ __forceinline__ void run_steps_ct(cplx<T>* lds, const PassArgs<T>& a, const TileGeo& g,
                                              int tid, const cplx<T>* wc)
 {
@@ -381,8 +385,8 @@ __device__ __forceinline__ void run_steps_ct(cplx<T>* lds, const PassArgs<T>& a,
         constexpr int q = as * RB;
         constexpr int rho = (as < full) ? RB : rem;
         __syncthreads();
-        radix_step<T, MODE, E, rho, NT, DRV>(lds, a, g, q, tid, wc);
-        run_steps_ct<T, MODE, E, NT, K, RB, S + 1, SEND, DRV>(lds, a, g, tid, wc);
+        radix_step<T, MODE, E, rho, NT, DRV, TW>(lds, a, g, q, tid, wc);
+        run_steps_ct<T, MODE, E, NT, K, RB, S + 1, SEND, DRV, TW>(lds, a, g, tid, wc);
     } else {
         __syncthreads();
     }

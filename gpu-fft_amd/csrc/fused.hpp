@@ -35,6 +35,7 @@ namespace gfa {
 enum ProdLoad {
     PL_FOLD = 0,       // T inputs [batch][2^(n+1)]: fold + twist
     PL_FOLD_LIMB = 1,  // int32 inputs: balanced base-2^w digit, fold + twist
+    PL_REAL = 2,       // T inputs [batch][2^n]: real part, no twist (R_R forward)
 };
 
 template <typename T> struct ProdArgs {
@@ -86,7 +87,7 @@ __device__ __forceinline__ float limb_digit(int v, int w, int i, int k, int o, i
 // 4 still hold the 2^HB partners of the top step, else 8-byte loads of 2.
 template <typename T, int PL, int E, int HB> constexpr int in_vec()
 {
-    return PL == PL_FOLD ? 2 : ((E - 2) >= HB ? 4 : 2);
+    return (sizeof(T) == 8 && PL != PL_FOLD_LIMB) ? 2 : ((E - 2) >= HB ? 4 : 2);
 }
 
 // Folded inputs of V consecutive positions P .. P+V-1 of this product's
@@ -98,12 +99,30 @@ template <typename T, int PL, int V>
 __device__ __forceinline__ void prod_input_vec(rsrc_t x, int L, int w, int limb, int k, long long o,
                                                int center, int P, T* re, T* im)
 {
-    if constexpr (PL == PL_FOLD) {
-        static_assert(sizeof(T) == 8 && V == 2, "the FP64 form reads doubles in pairs");
-        const u32x4 r = __builtin_amdgcn_raw_buffer_load_b128(x, (unsigned)P * 8u, 0, 0);
-        const u32x4 i = __builtin_amdgcn_raw_buffer_load_b128(x, (unsigned)(P + L) * 8u, 0, 0);
-        __builtin_memcpy(re, &r, 16);
-        __builtin_memcpy(im, &i, 16);
+    if constexpr (PL == PL_FOLD || PL == PL_REAL) {
+        // V consecutive T of 16 bytes (2 doubles, 4 floats) or 8 bytes (2 floats)
+        static_assert(V * sizeof(T) == 16 || V * sizeof(T) == 8, "16- or 8-byte loads");
+        if constexpr (V * sizeof(T) == 16) {
+            const u32x4 r = __builtin_amdgcn_raw_buffer_load_b128(x, (unsigned)P * (unsigned)sizeof(T), 0, 0);
+            __builtin_memcpy(re, &r, 16);
+            if constexpr (PL == PL_FOLD) {
+                const u32x4 i = __builtin_amdgcn_raw_buffer_load_b128(
+                    x, (unsigned)(P + L) * (unsigned)sizeof(T), 0, 0);
+                __builtin_memcpy(im, &i, 16);
+            }
+        } else {
+            const u32x2 r = __builtin_amdgcn_raw_buffer_load_b64(x, (unsigned)P * (unsigned)sizeof(T), 0, 0);
+            __builtin_memcpy(re, &r, 8);
+            if constexpr (PL == PL_FOLD) {
+                const u32x2 i = __builtin_amdgcn_raw_buffer_load_b64(
+                    x, (unsigned)(P + L) * (unsigned)sizeof(T), 0, 0);
+                __builtin_memcpy(im, &i, 8);
+            }
+        }
+        if constexpr (PL == PL_REAL) {
+#pragma unroll
+            for (int v = 0; v < V; v++) im[v] = T(0);
+        }
     } else if constexpr (V == 4) {
         const u32x4 r = __builtin_amdgcn_raw_buffer_load_b128(x, (unsigned)P * 4u, 0, 0);
         const u32x4 i = __builtin_amdgcn_raw_buffer_load_b128(x, (unsigned)(P + L) * 4u, 0, 0);
@@ -156,7 +175,7 @@ __device__ __forceinline__ cplx<T> top_forward_row(cplx<T>* zz, const cplx<T>* w
 // as twist[j 2^S + c] = twist[c] * twist[j 2^S] (up to rounding), so
 //     y_q[c] = twist[c] * sum_j C[j] z_j,  C[j] = row_q(e_j) * twist[j 2^S],
 // with uniform coefficients C computed once per workgroup.
-template <typename T, int SB>
+template <typename T, int SB, bool TWIST = true>
 __device__ __forceinline__ void top_row_coeffs(const ProdArgs<T>& a, int q, cplx<T>* C)
 {
     constexpr int Q = 1 << SB;
@@ -167,7 +186,7 @@ __device__ __forceinline__ void top_row_coeffs(const ProdArgs<T>& a, int q, cplx
         cplx<T> e[Q];
 #pragma unroll
         for (int k = 0; k < Q; k++) e[k] = mk<T>(T(0), T(0));
-        e[j] = a.twist[j << a.S];
+        e[j] = TWIST ? a.twist[j << a.S] : mk<T>(T(1), T(0));
         C[j] = top_forward_row<T, SB>(e, wq);
     }
 }
@@ -285,10 +304,18 @@ __device__ __forceinline__ void load_sub(cplx<T>* lds, const ProdArgs<T>& a, rsr
                 T ax = T(0), ay = T(0);
 #pragma unroll
                 for (int j = 0; j < Q; j++) {
-                    ax = ax + C[j].x * re[j][v] - C[j].y * im[j][v];
-                    ay = ay + C[j].x * im[j][v] + C[j].y * re[j][v];
+                    if constexpr (PL == PL_REAL) {  // imaginary parts are 0, no twist
+                        ax = ax + C[j].x * re[j][v];
+                        ay = ay + C[j].y * re[j][v];
+                    } else {
+                        ax = ax + C[j].x * re[j][v] - C[j].y * im[j][v];
+                        ay = ay + C[j].x * im[j][v] + C[j].y * re[j][v];
+                    }
                 }
-                y[v][m] = twist_of<T, S>(tws, c0 + v) * mk<T>(ax, ay);
+                if constexpr (PL == PL_REAL)
+                    y[v][m] = mk<T>(ax, ay);
+                else
+                    y[v][m] = twist_of<T, S>(tws, c0 + v) * mk<T>(ax, ay);
             }
         }
 #pragma unroll
@@ -303,7 +330,7 @@ __device__ __forceinline__ void load_sub(cplx<T>* lds, const ProdArgs<T>& a, rsr
 
 // Last forward step (bits [0, RB)) of the thread's 2^(E-RB) groups, into
 // v[k 2^RB + j] = position ((tid + k NT) << RB) | j.
-template <typename T, int E, int RB, int NT>
+template <typename T, int E, int RB, int NT, bool TW = false>
 __device__ __forceinline__ void last_forward_step(const cplx<T>* lds, const PassArgs<T>& pf,
                                                   const TileGeo& g, int tid, cplx<T>* v,
                                                   const cplx<T>* wc)
@@ -314,7 +341,7 @@ __device__ __forceinline__ void last_forward_step(const cplx<T>* lds, const Pass
         cplx<T> w[1 << RB];
 #pragma unroll
         for (int j = 0; j < (1 << RB); j++) w[j] = lds[swz(rbase | j)];
-        group_stages_derived<T, MODE_STD_FWD, RB>(w, pf, g, rbase, 0, 0, wc);
+        group_stages_derived<T, MODE_STD_FWD, RB, TW>(w, pf, g, rbase, 0, 0, wc);
 #pragma unroll
         for (int j = 0; j < (1 << RB); j++) v[(k << RB) + j] = w[j];
     }

@@ -1,178 +1,257 @@
-// One-launch transforms for the out-of-place drop-in calls (R_R and FFNT,
-// both directions) of length 2^n with n = S + SB, S the LDS tile (2^13
-// double / 2^14 float points) and SB = 1..3.
+// Transforms of 2^n points, n = S + SB (S: the LDS tile, 2^13 double /
+// 2^14 float points; SB = 1..3), for the out-of-place drop-in calls (R_R and
+// FFNT): the forward in one launch, the inverse in two.
 //
 // A transform is cut into Q = 2^SB sub-problems of 2^S points, one workgroup
-// each; a workgroup reads the whole transform (the Q workgroups of a
+// each.  A forward workgroup reads the whole transform (the Q workgroups of a
 // transform get block ids congruent mod 8, so they run on one XCD, close in
-// time, and the repeated reads are served by its L2) and evaluates the Q-point
-// network that couples the sub-problems in registers, for its own row only:
-//   forward (Cooley-Tukey, natural in, bit-reversed out): the couplings are
-//     the first SB stages (bits n-1 .. S), between positions j 2^S + c, and
-//     sub-problem q owns the output block [q 2^S, (q+1) 2^S); the row
-//     coefficients are uniform over the workgroup;
-//   inverse (Gentleman-Sande, bit-reversed in, natural out): the couplings
-//     are the first SB stages (bits 0 .. SB-1), between the adjacent
-//     positions (u << SB) | v, v < Q, and sub-problem r owns the outputs
-//     j = (u << SB) | r, u < 2^S (every Q-th output); the stage twiddles
-//     depend on u (one table load per distinct twiddle).
-// The remaining S stages of a sub-problem are the LDS radix steps of the pass
-// engine.  HBM sees every input once and every output once, where the pass
-// chain of the same transform reads and writes the whole transform twice.
-// Stage order and twiddle indices are those of the reference kernels
-// (ForwardCore / InverseCore, src/lib/fft.cu:11-260, and the FFNT cores,
-// fft.cu:1013-1384), so results match the pass chain up to rounding.
-// In-place calls (C_C, special) keep the pass chain: a workgroup here reads
-// positions that its siblings write.
+// time, and the repeated reads are served by its L2) and evaluates, for its
+// own row only, the Q-point network that couples the sub-problems:
+// the forward (Cooley-Tukey, natural in, bit-reversed out) couplings are the
+// first SB stages (bits n-1 .. S), between positions j 2^S + c, with
+// coefficients uniform over the workgroup (the fused products' gather,
+// fused.hpp); sub-problem q owns the output block [q 2^S, (q+1) 2^S).
+// The other S stages of a sub-problem are radix-2^RB steps through LDS with
+// twiddles derived from one table load per step (in-step roots tab[t], see
+// group_stages_derived); the short top step runs in registers in the gather.
+// HBM sees every input once and every output once.
+//
+// The inverse (Gentleman-Sande, bit-reversed in, natural out) is two
+// launches: the stages inside each contiguous sub-problem (its bits 0 .. S-1)
+// in place, then the SB coupling stages with the output transform as a
+// streaming pass.  (A one-launch inverse, partitioned by the low bits of the
+// output index so that its coupling stages come first, writes every Q-th
+// output, and those partial-line stores cost more than the second pass.)
+// Stage order and twiddle indices are
+// those of the reference kernels (ForwardCore / InverseCore,
+// src/lib/fft.cu:11-260, and the FFNT cores, fft.cu:1013-1384), so results
+// match the pass chain up to rounding.
 #pragma once
 
-#include "engine.hpp"
+#include "fused.hpp"
 
 namespace gfa {
 
-// Row q of the top SB forward stages as Q coefficients: y_q[c] = sum_j C[j] x_j[c],
-// x_j[c] the input at position j 2^S + c.  Stage i (bit S+i, top first)
-// keeps z[j] + z[j + 2^i] * wq[i], wq[i] = +-tab[base + (q >> (i+1))] (the
-// sign is bit i of q).
-template <typename T, int SB>
-__device__ __forceinline__ void oneshot_row_coeffs(const cplx<T>* tab, int n, int S, int nega,
-                                                   int q, cplx<T>* C)
+template <typename T> struct OneArgs {
+    int n;       // log2 transform length
+    int batch;   // transforms (products) in the launch
+    int nega;    // X^N+1 table layout (base offsets)
+    const cplx<T>* tab;      // root table of this direction (reference layout)
+    const cplx<T>* twist;    // PL_FOLD: FFNT twist table (2^n entries)
+    const void* src;
+    const void* src2;        // second operand (LD_MUL)
+    long long src_stride;    // elements between transforms
+    void* dst;
+    long long dst_stride;
+    const cplx<T>* untwist;  // ST_UNFOLD: untwist table (2^n entries)
+    int do_scale;
+    cplx<T> scale;
+};
+
+// XCD-aware block order: the Q workgroups of a transform get block ids
+// congruent mod 8 (consecutive ids go to consecutive XCDs).
+template <int Q> __device__ __forceinline__ void one_block(int& row, long long& t)
 {
-    constexpr int Q = 1 << SB;
-    cplx<T> wq[SB];
-#pragma unroll
-    for (int i = 0; i < SB; i++) {
-        const int b = S + i;
-        const int base = nega ? (1 << (n - 1 - b)) : 0;
-        cplx<T> w = tab[base + (q >> (i + 1))];
-        if ((q >> i) & 1) w = mk<T>(-w.x, -w.y);
-        wq[i] = w;
-    }
-#pragma unroll
-    for (int j = 0; j < Q; j++) {
-        cplx<T> z[Q];
-#pragma unroll
-        for (int k = 0; k < Q; k++) z[k] = mk<T>(T(j == k), T(0));
-#pragma unroll
-        for (int i = SB - 1; i >= 0; i--) {
-#pragma unroll
-            for (int k = 0; k < (1 << i); k++) z[k] = z[k] + z[k + (1 << i)] * wq[i];
-        }
-        C[j] = z[0];
-    }
+    const int x = blockIdx.x;
+    row = (x >> 3) % Q;
+    t = (long long)((x >> 3) / Q) * 8 + (x & 7);
 }
 
-// io: the call's input/output description with io.n = n (fold/unfold
-// offsets, output twiddle positions); stages use the same table.
-template <typename T, int E, int SB, int LK, int SK>
-__global__ __launch_bounds__(512) void oneshot_forward_kernel(PassArgs<T> io)
+template <typename T, int S, int SB, int E, int RB, int PL>
+__global__ __launch_bounds__((1 << S) >> E) void fwd_sub_kernel(OneArgs<T> o)
 {
-    constexpr int NT = 512;
+    constexpr int TILE = 1 << S;
     constexpr int EPT = 1 << E;
-    constexpr int TILE = NT * EPT;
-    constexpr int S = E + 9;
+    constexpr int NT = TILE / EPT;
     constexpr int Q = 1 << SB;
+    constexpr bool TWIST = PL != PL_REAL;
+    constexpr int NST = (S + RB - 1) / RB;
+    constexpr int HB = S % RB;
+    constexpr int FS0 = HB ? 1 : 0;
     __shared__ cplx<T> lds[TILE];
+    __shared__ cplx<T> tws[TWIST ? TwistSplit<S>::NHI + TwistSplit<S>::NLO : 1];
     const int tid = threadIdx.x;
-    const int x = blockIdx.x;
-    const int q = (x >> 3) % Q;
-    const long long t = (long long)((x >> 3) / Q) * 8 + (x & 7);
-    if (t >= io.batch) return;
+    int q;
+    long long t;
+    one_block<Q>(q, t);
+    if (t >= o.batch) return;
 
-    cplx<T> C[Q];
-    oneshot_row_coeffs<T, SB>(io.tab, io.n, S, io.nega, q, C);
-#pragma unroll 2
-    for (int i = 0; i < EPT; i++) {
-        const int c = tid + i * NT;
-        cplx<T> xv[Q];
-#pragma unroll
-        for (int j = 0; j < Q; j++) xv[j] = load_elem<T, LK>(io, t, (j << S) | c);
-        cplx<T> y = C[0] * xv[0];
-#pragma unroll
-        for (int j = 1; j < Q; j++) y = y + C[j] * xv[j];
-        lds[swz(c)] = y;
-    }
-    PassArgs<T> st = io;
-    st.blo = 0;
-    st.K = S;
-    st.logC = 0;
+    ProdArgs<T> a{};
+    a.n = o.n;
+    a.S = S;
+    a.nega = o.nega;
+    a.tab = o.tab;
+    a.twist = o.twist;
+    PassArgs<T> pf{};
+    pf.n = o.n;
+    pf.K = S;
+    pf.nega = o.nega;
+    pf.tab = o.tab;
     TileGeo g;
     g.hi_pos = q << S;
     g.blo = 0;
     g.K = S;
     g.logC = 0;
-    g.tab = make_rsrc(io.tab);
-    run_steps<T, MODE_STD_FWD, E, NT>(lds, st, g, tid, 0, num_steps(S, E));
+    g.tab = make_rsrc(o.tab);
+    if constexpr (TWIST) fill_twist_split<T, S>(tws, o.twist, tid, NT);  // read after a barrier
+    cplx<T> C[Q];
+    top_row_coeffs<T, SB, TWIST>(a, q, C);
+    const rsrc_t xa = make_rsrc(reinterpret_cast<const T*>(o.src) + t * o.src_stride);
+    __syncthreads();
+    load_sub<T, S, SB, E, PL, HB>(lds, a, xa, tws, 0, 0, 1, 0, 0, false, C, q, tid);
+    run_steps_ct<T, MODE_STD_FWD, E, NT, S, RB, FS0, NST, true, true>(lds, pf, g, tid, o.tab);
+    cplx<T>* dst = reinterpret_cast<cplx<T>*>(o.dst) + t * o.dst_stride + ((long long)q << S);
 #pragma unroll 4
     for (int i = 0; i < EPT; i++) {
         const int c = tid + i * NT;
-        store_elem<T, SK>(io, t, (q << S) | c, lds[swz(c)]);
+        stc_nt(dst + c, lds[swz(c)]);
     }
 }
 
-template <typename T, int E, int SB, int LK, int SK>
-__global__ __launch_bounds__(512) void oneshot_inverse_kernel(PassArgs<T> io)
+// Inverse, first launch: the Gentleman-Sande stages on bits 0 .. S-1 of
+// sub-problem q (the contiguous block [q 2^S, (q+1) 2^S) of a bit-reversed
+// spectrum, so its stages need nothing outside it), in place in the source
+// buffer (the reference's inverse also works in place in temp,
+// fft.cu:1456-1510); with LD_MUL the block is the product of src and src2
+// and goes to src.  The short top step (HB bits) runs in registers before
+// the stores.
+template <typename T, int S, int E, int RB, int LK>
+__global__ __launch_bounds__((1 << S) >> E) void inv_local_kernel(OneArgs<T> o)
 {
-    constexpr int NT = 512;
+    constexpr int TILE = 1 << S;
     constexpr int EPT = 1 << E;
-    constexpr int TILE = NT * EPT;
-    constexpr int S = E + 9;
-    constexpr int Q = 1 << SB;
+    constexpr int NT = TILE / EPT;
+    constexpr int NST = (S + RB - 1) / RB;
+    constexpr int HB = S % RB;
+    constexpr int IEND = HB ? NST - 1 : NST;
     __shared__ cplx<T> lds[TILE];
     const int tid = threadIdx.x;
-    const int x = blockIdx.x;
-    const int r = (x >> 3) % Q;
-    const long long t = (long long)((x >> 3) / Q) * 8 + (x & 7);
-    if (t >= io.batch) return;
-    const rsrc_t tab = make_rsrc(io.tab);
-    const int n = io.n;
-
-#pragma unroll 2
+    const int Q = 1 << (o.n - S);
+    const int q = blockIdx.x & (Q - 1);
+    const long long t = blockIdx.x >> (o.n - S);
+    if (t >= o.batch) return;
+    cplx<T>* x = reinterpret_cast<cplx<T>*>(const_cast<void*>(o.src)) + t * o.src_stride +
+                 ((long long)q << S);
+    const cplx<T>* y = LK == LD_MUL ? reinterpret_cast<const cplx<T>*>(o.src2) + t * o.src_stride +
+                                          ((long long)q << S)
+                                    : nullptr;
+#pragma unroll 4
     for (int i = 0; i < EPT; i++) {
-        const int u = tid + i * NT;
-        cplx<T> z[Q];
-#pragma unroll
-        for (int v = 0; v < Q; v++) z[v] = load_elem<T, LK>(io, t, (u << SB) | v);
-        // stages on bits 0 .. SB-1 of p = (u << SB) | v, twiddle
-        // tab[base_b + (p >> (b+1))] (reference InverseCore, fft.cu:154-260)
-#pragma unroll
-        for (int b = 0; b < SB; b++) {
-            const int base = io.nega ? (1 << (n - 1 - b)) : 0;
-#pragma unroll
-            for (int v0 = 0; v0 < Q; v0++) {
-                if (v0 & (1 << b)) continue;
-                const int v1 = v0 | (1 << b);
-                const cplx<T> w = bldc<T>(tab, base + (((u << SB) | v0) >> (b + 1)));
-                const cplx<T> U = z[v0], V = z[v1];
-                z[v0] = U + V;
-                z[v1] = (U - V) * w;
-            }
-        }
-        cplx<T> y = z[0];
-#pragma unroll
-        for (int v = 1; v < Q; v++)
-            if (r == v) y = z[v];
-        lds[swz(u)] = y;
+        const int c = tid + i * NT;
+        cplx<T> v = ldc(x + c);
+        if constexpr (LK == LD_MUL) v = v * ldc(y + c);
+        lds[swz(c)] = v;
     }
-    // the remaining stages (bits SB .. n-1 of p = bits 0 .. S-1 of u) are
-    // those of a transform of 2^S points: p >> (b+1) = u >> (b-SB+1) and
-    // base 2^(n-1-b) = 2^(S-1-(b-SB))
-    PassArgs<T> st = io;
-    st.n = S;
-    st.blo = 0;
-    st.K = S;
-    st.logC = 0;
+    PassArgs<T> pi{};
+    pi.n = o.n;
+    pi.K = S;
+    pi.nega = o.nega;
+    pi.tab = o.tab;
     TileGeo g;
-    g.hi_pos = 0;
+    g.hi_pos = q << S;
     g.blo = 0;
     g.K = S;
     g.logC = 0;
-    g.tab = tab;
-    run_steps<T, MODE_STD_INV, E, NT>(lds, st, g, tid, 0, num_steps(S, E));
-#pragma unroll 4
-    for (int i = 0; i < EPT; i++) {
-        const int u = tid + i * NT;
-        store_elem<T, SK>(io, t, (u << SB) | r, lds[swz(u)]);
+    g.tab = make_rsrc(o.tab);
+    run_steps_ct<T, MODE_STD_INV, E, NT, S, RB, 0, IEND, true, true>(lds, pi, g, tid, o.tab);
+    ProdArgs<T> a{};
+    a.n = o.n;
+    a.S = S;
+    a.nega = o.nega;
+    a.itab = o.tab;
+    constexpr int NP = 1 << HB;
+    constexpr int NI = EPT / NP;
+#pragma unroll 2
+    for (int i = 0; i < NI; i++) {
+        const int c = tid + i * NT;
+        cplx<T> z[NP];
+#pragma unroll
+        for (int m = 0; m < NP; m++) z[m] = lds[swz(c + (m << (S - HB)))];
+        if constexpr (HB > 0) top_gs<T, S, HB>(z, a, q);
+#pragma unroll
+        for (int m = 0; m < NP; m++) stc(x + c + (m << (S - HB)), z[m]);
+    }
+}
+
+// Inverse, second launch: the top SB Gentleman-Sande stages (bits S .. n-1)
+// of two adjacent columns c, c+1 per thread from the 2^SB rows j 2^S + c of
+// the first launch's result, twiddle tab[base_b + (j0 >> (i+1))] (uniform:
+// the column bits are below the stage bit), then the scale and the output:
+// ST_REAL Re(z mod_inverse) (reference InverseCore R_R store,
+// fft.cu:499-514), ST_UNFOLD z mod_inverse untwist[p] unfolded to Re at p
+// and Im at p + L (InverseFFNTCoreUnique, fft.cu:1366-1383), with
+// untwist[p] = untwist[p & ~127] untwist[p & 127] from LDS.  Two adjacent
+// outputs per store.
+template <typename T, int SB, int SK>
+__global__ __launch_bounds__(256) void inv_top_kernel(OneArgs<T> o, int S)
+{
+    constexpr int Q = 1 << SB;
+    constexpr int UTN = SK == ST_UNFOLD ? (1 << (14 + 3 - 7)) + 128 : 1;
+    __shared__ cplx<T> ut[UTN];
+    const int tid = threadIdx.x;
+    const int n = o.n;
+    if constexpr (SK == ST_UNFOLD) {
+        const int nhi = 1 << (n - 7);
+        for (int i = tid; i < nhi + 128; i += 256)
+            ut[i] = i < nhi ? o.untwist[i << 7] : o.untwist[i - nhi];
+        __syncthreads();
+    }
+    const long long gid = (long long)blockIdx.x * 256 + tid;
+    const long long t = gid >> (S - 1);
+    const int c = (int)(gid & ((1LL << (S - 1)) - 1)) * 2;
+    if (t >= o.batch) return;
+    const long long L = 1LL << n;
+    const cplx<T>* src = reinterpret_cast<const cplx<T>*>(o.src) + t * o.src_stride;
+    cplx<T> w[SB][Q / 2];
+#pragma unroll
+    for (int i = 0; i < SB; i++) {
+        const int base = o.nega ? (1 << (n - 1 - (S + i))) : 0;
+#pragma unroll
+        for (int m = 0; m < Q / 2; m++)
+            if ((m << (i + 1)) < Q) w[i][m] = o.tab[base + m];
+    }
+    cplx<T> y[2][Q];
+#pragma unroll
+    for (int j = 0; j < Q; j++) {
+        y[0][j] = ldc_nt(src + ((long long)j << S) + c);
+        y[1][j] = ldc_nt(src + ((long long)j << S) + c + 1);
+    }
+#pragma unroll
+    for (int v = 0; v < 2; v++) {
+#pragma unroll
+        for (int i = 0; i < SB; i++) {
+#pragma unroll
+            for (int j0 = 0; j0 < Q; j0++) {
+                if (j0 & (1 << i)) continue;
+                const int j1 = j0 | (1 << i);
+                const cplx<T> U = y[v][j0], V = y[v][j1];
+                y[v][j0] = U + V;
+                y[v][j1] = (U - V) * w[i][j0 >> (i + 1)];
+            }
+        }
+    }
+    T* out = reinterpret_cast<T*>(o.dst) + t * o.dst_stride;
+    typedef T vec2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+    for (int j = 0; j < Q; j++) {
+        const int P = (j << S) | c;
+        cplx<T> z[2];
+#pragma unroll
+        for (int v = 0; v < 2; v++) {
+            z[v] = y[v][j] * o.scale;
+            if constexpr (SK == ST_UNFOLD)
+                z[v] = z[v] * (ut[(P + v) >> 7] * ut[(1 << (n - 7)) + ((P + v) & 127)]);
+        }
+        vec2 re;
+        re.x = z[0].x;
+        re.y = z[1].x;
+        __builtin_nontemporal_store(re, reinterpret_cast<vec2*>(out + P));
+        if constexpr (SK == ST_UNFOLD) {
+            vec2 im;
+            im.x = z[0].y;
+            im.y = z[1].y;
+            __builtin_nontemporal_store(im, reinterpret_cast<vec2*>(out + P + L));
+        }
     }
 }
 

@@ -180,81 +180,89 @@ int run_transform(int mode, int n, int batch, int nega, const cplx<T>* tab,
     return ok();
 }
 
-// One-launch path of the out-of-place calls (R_R, FFNT): a transform of
-// 2^n points, n = S + SB with S the LDS tile (13 double, 14 float) and
-// SB = 1..3, is one launch of Q = 2^SB workgroups per transform.
+// Out-of-place calls (R_R, FFNT) of 2^n points, n = S + SB with S the LDS
+// tile (13 double, 14 float) and SB = 1..3 (oneshot.hpp): the forward is one
+// launch of Q = 2^SB workgroups per transform; the inverse is a launch of
+// the in-sub-problem stages (in place in the source) and a streaming launch
+// of the top SB stages with the output transform.
 template <typename T> struct OneCfg;
 template <> struct OneCfg<double> {
-    static constexpr int E = 4;  // 16 points per thread, tile 2^13
+    static constexpr int S = 13, E = 4, RB = 4;  // 512 threads, 16 points each
 };
 template <> struct OneCfg<float> {
-    static constexpr int E = 5;  // 32 points per thread, tile 2^14
+    static constexpr int S = 14, E = 5, RB = 4;  // 512 threads, 32 points each
 };
 
-template <typename T, int SB> PassFn<T> select_oneshot_sb(int mode, int lk, int sk)
+template <typename T> using OneFn = void (*)(OneArgs<T>);
+template <typename T> using TopFn = void (*)(OneArgs<T>, int);
+
+template <typename T, int SB> OneFn<T> select_forward_sb(int lk)
 {
-    constexpr int E = OneCfg<T>::E;
-#define GFA_ONE_F(L, S) \
-    if (mode == MODE_STD_FWD && lk == L && sk == S) return &oneshot_forward_kernel<T, E, SB, L, S>;
-#define GFA_ONE_I(L, S) \
-    if (mode == MODE_STD_INV && lk == L && sk == S) return &oneshot_inverse_kernel<T, E, SB, L, S>;
-    GFA_ONE_F(LD_REAL, ST_CPLX)
-    GFA_ONE_F(LD_FOLD, ST_CPLX)
-    GFA_ONE_I(LD_CPLX, ST_REAL)
-    GFA_ONE_I(LD_MUL, ST_REAL)
-    GFA_ONE_I(LD_CPLX, ST_UNFOLD)
-    GFA_ONE_I(LD_MUL, ST_UNFOLD)
-#undef GFA_ONE_F
-#undef GFA_ONE_I
+    constexpr int S = OneCfg<T>::S, E = OneCfg<T>::E, RB = OneCfg<T>::RB;
+    if (lk == LD_REAL) return &fwd_sub_kernel<T, S, SB, E, RB, PL_REAL>;
+    if (lk == LD_FOLD) return &fwd_sub_kernel<T, S, SB, E, RB, PL_FOLD>;
     return nullptr;
 }
 
-// Returns true and launches when the one-launch path covers this transform
-// (status in *status); false leaves the pass chain to the caller.  The
-// source must not overlap the destination.
+template <typename T, int SB> TopFn<T> select_top_sb(int sk)
+{
+    if (sk == ST_REAL) return &inv_top_kernel<T, SB, ST_REAL>;
+    if (sk == ST_UNFOLD) return &inv_top_kernel<T, SB, ST_UNFOLD>;
+    return nullptr;
+}
+
+// Returns true and launches when this path covers the transform (status in
+// *status); false leaves the pass chain to the caller.  The source must not
+// overlap the destination; the inverse uses the source as its work buffer.
 template <typename T>
 bool run_oneshot(int mode, int n, int batch, int nega, const cplx<T>* tab,
                  const TransformIO<T>& io, hipStream_t stream, const char* who, int* status)
 {
-    constexpr int S = OneCfg<T>::E + 9;
+    constexpr int S = OneCfg<T>::S, E = OneCfg<T>::E, RB = OneCfg<T>::RB;
     const int SB = n - S;
     if (SB < 1 || SB > 3) return false;
-    PassFn<T> fn = nullptr;
-    switch (SB) {
-        case 1: fn = select_oneshot_sb<T, 1>(mode, io.lk, io.sk); break;
-        case 2: fn = select_oneshot_sb<T, 2>(mode, io.lk, io.sk); break;
-        case 3: fn = select_oneshot_sb<T, 3>(mode, io.lk, io.sk); break;
-    }
-    if (!fn) return false;
-    PassArgs<T> a{};
+    OneArgs<T> a{};
     a.n = n;
-    a.blo = 0;
-    a.K = S;
-    a.logC = 0;
-    a.logBT = 0;
     a.batch = batch;
     a.nega = nega;
     a.tab = tab;
+    a.twist = io.ltab;
     a.src = io.src;
     a.src2 = io.src2;
-    a.ltab = io.ltab;
     a.src_stride = io.src_stride;
     a.dst = io.dst;
-    a.stab = io.stab;
     a.dst_stride = io.dst_stride;
+    a.untwist = io.stab;
     a.do_scale = io.do_scale ? 1 : 0;
-    a.scale = io.scale;
-    // the sibling workgroups of a transform re-read its input from L2:
-    // cached loads; every output is written once (the inverse writes every
-    // Q-th output, so its lines fill up from Q workgroups in L2: cached)
-    a.nt_in = 0;
-    a.nt_out = mode == MODE_STD_FWD ? 1 : 0;
-    const long long blocks = (((long long)batch + 7) / 8) * 8 * (1LL << SB);
-    if (blocks > 0x7fffffffLL) {
-        *status = fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": batch too large");
-        return true;
+    a.scale = io.do_scale ? io.scale : mk<T>(T(1), T(0));
+    if (mode == MODE_STD_FWD && io.sk == ST_CPLX && (io.lk == LD_REAL || io.lk == LD_FOLD)) {
+        OneFn<T> fn = SB == 1 ? select_forward_sb<T, 1>(io.lk)
+                    : SB == 2 ? select_forward_sb<T, 2>(io.lk)
+                              : select_forward_sb<T, 3>(io.lk);
+        const long long blocks = (((long long)batch + 7) / 8) * 8 * (1LL << SB);
+        if (blocks > 0x7fffffffLL) {
+            *status = fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": batch too large");
+            return true;
+        }
+        hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3((1 << S) >> E), 0, stream, a);
+    } else if (mode == MODE_STD_INV && (io.lk == LD_CPLX || io.lk == LD_MUL) &&
+               (io.sk == ST_REAL || io.sk == ST_UNFOLD)) {
+        const long long blocks1 = (long long)batch << SB;
+        const long long blocks2 = (((long long)batch << (S - 1)) + 255) / 256;
+        if (blocks1 > 0x7fffffffLL || blocks2 > 0x7fffffffLL) {
+            *status = fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": batch too large");
+            return true;
+        }
+        OneFn<T> f1 = io.lk == LD_MUL ? &inv_local_kernel<T, S, E, RB, LD_MUL>
+                                      : &inv_local_kernel<T, S, E, RB, LD_CPLX>;
+        hipLaunchKernelGGL(f1, dim3((unsigned)blocks1), dim3((1 << S) >> E), 0, stream, a);
+        TopFn<T> f2 = SB == 1 ? select_top_sb<T, 1>(io.sk)
+                    : SB == 2 ? select_top_sb<T, 2>(io.sk)
+                              : select_top_sb<T, 3>(io.sk);
+        hipLaunchKernelGGL(f2, dim3((unsigned)blocks2), dim3(256), 0, stream, a, S);
+    } else {
+        return false;
     }
-    hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(512), 0, stream, a);
     hipError_t e = hipGetLastError();
     *status = e != hipSuccess ? fail_hip(e, who) : ok();
     return true;

@@ -48,21 +48,26 @@ def test_gpu_tables_match_reference_layout(n_power):
 
 
 # ---- FP64 fused product ----
-@pytest.mark.parametrize("combine", ["in_kernel", "pass"])
+# rings below 2^15 are one workgroup per product (no combine step); from
+# 2^15 the product spans 2^(ring_power-14) workgroups and its last inverse
+# stages run in the product kernel ("in_kernel") or in a second launch
+# ("pass")
+F64_CASES = ([(r, "single") for r in (7, 8, 12, 14)] +
+             [(r, c) for r in (15, 16, 17) for c in ("in_kernel", "pass")])
+
+
 @pytest.mark.parametrize("rb", [3, 4])
-@pytest.mark.parametrize("ring_power", [7, 8, 12, 14, 15, 16, 17])
-def test_polymul_f64_exact(ring_power, rb, combine):
+@pytest.mark.parametrize("ring_power,combine", F64_CASES)
+def test_polymul_f64_exact(ring_power, combine, rb):
     G = _g()
     import torch
-    if combine == "pass" and ring_power < 15:
-        pytest.skip("one workgroup per product: no combine step")
     N = 1 << ring_power
     batch = 11  # not a multiple of the 8-XCD block grouping
     a, b = _operands(N, batch, SEED_BASE + 3 + ring_power)
     plan = G.PolymulPlan(ring_power, G.PREC_F64, max_batch=batch)
     plan.radix_bits = rb
     plan.combine_in_kernel = combine == "in_kernel"
-    assert plan.combine_in_kernel == (combine == "in_kernel" and ring_power >= 15)
+    assert plan.combine_in_kernel == (combine == "in_kernel")
     out = torch.empty(batch * N, dtype=torch.float64, device="cuda")
     plan.multiply_f64(to_dev(a.astype(np.float64).reshape(-1)), to_dev(b.astype(np.float64).reshape(-1)),
                       out, batch)
