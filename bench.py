@@ -296,7 +296,41 @@ def batched_uneven(args):
             for j, i in enumerate(idx):
                 o0 = descs[i].out_offset
                 exact = exact and np.array_equal(got[o0:o0 + olen], res[j, :olen])
-    print(json.dumps({"workload": "batched uneven products", "count": count,
+    cpu = {}
+    if not args.no_cpu:
+        # the reference's CPU paths per product: FFT<double>::fft/ifft of the
+        # zero-padded operands for the linear products, FFNT<double>::MULT for
+        # the negacyclic ones (restated in C); each (kind, ring size) group is
+        # timed on a sample and scaled by its product count
+        import oracle as O
+        P = _host_threads()
+        t1 = tp = 0.0
+        for (k, r), idx in sorted(groups.items()):
+            N = 1 << r
+            pick = [idx[i % len(idx)] for i in range(P)]
+            xa = np.stack([a[descs[i].a_offset:descs[i].a_offset + N] for i in pick]).astype(np.int64)
+            xb = np.stack([b[descs[i].b_offset:descs[i].b_offset + N] for i in pick]).astype(np.int64)
+            fn = O.cpu_fft_mult if k == G.PRODUCT_LINEAR else O.cpu_ffnt_mult
+            fn(xa[:1], xb[:1], threads=1)
+            reps = max(1, int(0.05 / max(1e-6, 1e-7 * N * r)))
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn(xa[:1], xb[:1], threads=1)
+            t1 += len(idx) * (time.perf_counter() - t0) / reps
+            t0 = time.perf_counter()
+            fn(xa, xb, threads=P)
+            tp += len(idx) * (time.perf_counter() - t0) / P
+        model = _cpu_model()
+        what = ("every (kind, ring size) group sampled and scaled by its count: FFT<double>::fft/"
+                "ifft products (linear) and FFNT<double>::MULT (negacyclic), restated in C")
+        cpu = {"cpu_baseline": {"value": count / t1, "unit": "products/s", "cores": 1,
+                                "kind": "port", "sample": what + ", one thread",
+                                "list_seconds": t1, "cpu_model": model},
+               "cpu_baseline_parallel": {"value": count / tp, "unit": "products/s", "cores": P,
+                                         "kind": "port",
+                                         "sample": what + ", %d threads, one product per task" % P,
+                                         "list_seconds": tp, "cpu_model": model}}
+    print(json.dumps({**cpu, "workload": "batched uneven products", "count": count,
                       "ring_powers": "10..17", "kinds": "half linear (C_C), half negacyclic",
                       "coefficients": "[-2^8, 2^8)", "fp32_split": "a, b: 2 limbs of 5 bits",
                       "tuned": list(cand),
@@ -329,6 +363,36 @@ def _poly_check(a, b, c, p=2147483647, points=3, seed=1):
     xs = np.random.default_rng(seed).integers(2, p - 1, size=points).tolist()
     ea, eb, ec = _eval_mod(a, xs, p), _eval_mod(b, xs, p), _eval_mod(c, xs, p)
     return bool(np.array_equal((ea * eb) % p, ec))
+
+
+def _host_threads():
+    """Host cores available to this process, at most 16 (the host share of
+    one GPU on the boxes this runs on)."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def _cpu_rates(run, seconds, unit, what):
+    """CPU baseline of one configuration on this host: run(threads, m) does m
+    units of work on `threads` threads; one thread for about `seconds`, then
+    batch-parallel on every available host core for about seconds / 2."""
+    model = _cpu_model()
+    out = {}
+    for key, threads, secs in (("cpu_baseline", 1, seconds),
+                               ("cpu_baseline_parallel", _host_threads(), seconds / 2)):
+        m = threads
+        run(threads, m)  # warm-up (tables, pages)
+        t0 = time.perf_counter()
+        cnt = 0
+        while True:
+            run(threads, m)
+            cnt += m
+            if time.perf_counter() - t0 > secs:
+                break
+        dt = time.perf_counter() - t0
+        out[key] = {"value": cnt / dt, "unit": unit, "cores": threads, "kind": "port",
+                    "sample": "%d %s, %d thread(s), %.1f s" % (cnt, what, threads, dt),
+                    "cpu_model": model}
+    return out
 
 
 def config_suite(args):
@@ -397,8 +461,32 @@ def config_suite(args):
         got = np.round(buf[:L].real.cpu().numpy()).astype(np.int64)
         exact = np.array_equal(got[:2 * n - 1], np.convolve(a, b)) and not got[2 * n - 1:].any()
         t_cc = graph_ms(cc)
-        print(json.dumps({"config": "C_C cyclic product, degree < 2^12, [0,16), zero-padded 2^13, f64",
-                          "gpu_us_per_product": t_cc * 1e3, "bit_exact": bool(exact)}), flush=True)
+        # the same product as one launch pair of the batched plan: a LINEAR
+        # product of ring 2^12 (negacyclic of ring 2^13, FP32 split, 2 limbs
+        # of 5 bits per operand), graph-replayed
+        descs, n_in, n_out = G.pack_products([G.PRODUCT_LINEAR], [12])
+        bp = G.BatchedPlan(descs, 5, 2, 5, 2, stream=sid)
+        da = torch.from_numpy(a.astype(np.int32)).to(dev)
+        db = torch.from_numpy(b.astype(np.int32)).to(dev)
+        bout = torch.zeros(n_out, dtype=torch.int64, device=dev)
+        bp.graph_launch(da, db, bout, stream=sid)
+        torch.cuda.synchronize()
+        gb = bout.cpu().numpy()
+        exact_b = np.array_equal(gb[:2 * n - 1], np.convolve(a, b)) and not gb[2 * n - 1:].any()
+        t_b = ev_ms(lambda: bp.graph_launch(da, db, bout, stream=sid), 50)
+        bp.close()
+        line = {"config": "C_C cyclic product, degree < 2^12, [0,16), zero-padded 2^13, f64",
+                "gpu_us_per_product": t_cc * 1e3, "bit_exact": bool(exact),
+                "batched_linear_fp32split_us": t_b * 1e3, "batched_linear_bit_exact": bool(exact_b)}
+        if not args.no_cpu:
+            import oracle as O
+            ra = np.tile(a, (_host_threads(), 1))
+            rb = np.tile(b, (_host_threads(), 1))
+            line.update(_cpu_rates(lambda th, m: O.cpu_fft_mult(ra[:m], rb[:m], threads=th),
+                                   args.cpu_seconds, "products/s",
+                                   "products of 4096 coefficients (FFT<double>::fft/ifft product "
+                                   "of example/test_fft_cpu.cu, restated in C)"))
+        print(json.dumps(line), flush=True)
 
         # ---- R_R batch 64, N = 2^16 (inputs degree < 2^15)
         B, N, h = 64, 1 << 16, 1 << 15
@@ -427,12 +515,19 @@ def config_suite(args):
         exact = _poly_check(a, b, prod)
         fwd_bytes = 2 * B * (N * 8 + N * 16)   # real in, full complex spectrum out
         mul_bytes = B * 3 * N * 8              # two real operands in, real product out
-        print(json.dumps({"config": "R_R cyclic products, N=2^16 (degree < 2^15), batch 64, "
-                                    "[-2^10,2^10), f64",
-                          "forward_ms": t_fwd, "forward_alg_GBps": fwd_bytes / (t_fwd * 1e-3) / 1e9,
-                          "multiply_ms": t_mul, "multiply_alg_GBps": mul_bytes / (t_mul * 1e-3) / 1e9,
-                          "products_per_s": B / (t_mul * 1e-3),
-                          "bit_exact_at_random_points_mod_2^31-1": exact}), flush=True)
+        line = {"config": "R_R cyclic products, N=2^16 (degree < 2^15), batch 64, "
+                          "[-2^10,2^10), f64",
+                "forward_ms": t_fwd, "forward_alg_GBps": fwd_bytes / (t_fwd * 1e-3) / 1e9,
+                "multiply_ms": t_mul, "multiply_alg_GBps": mul_bytes / (t_mul * 1e-3) / 1e9,
+                "products_per_s": B / (t_mul * 1e-3),
+                "bit_exact_at_random_points_mod_2^31-1": exact}
+        if not args.no_cpu:
+            import oracle as O
+            line.update(_cpu_rates(lambda th, m: O.cpu_fft_mult(a[:m], b[:m], threads=th),
+                                   args.cpu_seconds, "products/s",
+                                   "products of 2^15 coefficients zero-padded to 2^16 "
+                                   "(FFT<double>::fft/ifft product, restated in C)"))
+        print(json.dumps(line), flush=True)
         del rsrc, rbuf, temp
 
         # ---- special FFT, l = 2^15 slots, batch 512
@@ -468,13 +563,36 @@ def config_suite(args):
         e_rt = float((x - dec64).abs().max())
         e_rt64 = float((dec64 - zs).abs().max())
         bytes_tr = B * l * 16 * 2
-        print(json.dumps({"config": "special FFT, 2^15 slots x 2^30 on the unit disk, batch 512",
-                          "f64_encode_ms": t_enc64, "f64_decode_ms": t_dec64,
-                          "f64_alg_GBps": bytes_tr / (t_enc64 * 1e-3) / 1e9,
-                          "f32split_encode_ms": t_enc32, "f32split_decode_ms": t_dec32,
-                          "f32split_encode_rel_err_vs_f64": e_enc,
-                          "f32split_round_trip_max_slot_err": e_rt,
-                          "f64_round_trip_max_slot_err": e_rt64}), flush=True)
+        line = {"config": "special FFT, 2^15 slots x 2^30 on the unit disk, batch 512",
+                "f64_encode_ms": t_enc64, "f64_decode_ms": t_dec64,
+                "f64_alg_GBps": bytes_tr / (t_enc64 * 1e-3) / 1e9,
+                "f32split_encode_ms": t_enc32, "f32split_decode_ms": t_dec32,
+                "f32split_encode_rel_err_vs_f64": e_enc,
+                "f32split_round_trip_max_slot_err": e_rt,
+                "f64_round_trip_max_slot_err": e_rt64}
+        if not args.no_cpu:
+            # the reference has no CPU special FFT: the oracle's FP64 one (C),
+            # one encode + decode round trip per unit
+            import oracle as O
+            from concurrent.futures import ThreadPoolExecutor
+            Tc, Tic = O.special_table(l), O.special_table(l, True)
+            zc = z[:l].copy()
+
+            def rt_one(_):
+                u = O.special_inverse(zc, n, 1, Tic, complex(1.0 / l))
+                O.special_forward(u, n, 1, Tc)
+
+            def rt(th, m):
+                if th == 1:
+                    for i in range(m):
+                        rt_one(i)
+                else:
+                    with ThreadPoolExecutor(th) as ex:
+                        list(ex.map(rt_one, range(m)))
+            line.update(_cpu_rates(rt, args.cpu_seconds, "round trips/s",
+                                   "encode + decode round trips of 2^15 slots (the build's own FP64 "
+                                   "special FFT in C; the reference has no CPU special FFT)"))
+        print(json.dumps(line), flush=True)
 
 
 def _cpu_model():
@@ -615,13 +733,23 @@ def main():
             ms = timed(lambda: (io[: batch * N].copy_(a64), dropin()), stream, args.steps,
                        args.warmup)
             ms_copy = timed(lambda: io[: batch * N].copy_(a64), stream, args.steps, 1)
+            ms_fwd = timed(lambda: G.GPU_FFNT(io, temp, tw, rt, cf, 2 * batch), stream, args.steps, 1)
             io[: batch * N].copy_(a64)
             dropin()
             torch.cuda.synchronize()
             r = torch.round(io[: batch * N])
             d = (io[: batch * N] - r).abs()
-            res["f64_dropin"] = {"ms": ms - ms_copy, "err_max": float(d.max()),
-                                 "err_rms": float(torch.sqrt((d * d).mean()))}
+            # the two-call API moves at least 56N bytes per product: 2 x 8N real
+            # in and 16N spectra out (forward), 16N spectra in and 8N real out
+            # (inverse with product); 24N is the product's own input and output
+            api_bytes = 56 * N * batch
+            res["f64_dropin"] = {"ms": ms - ms_copy, "ms_forward": ms_fwd,
+                                 "ms_inverse": ms - ms_copy - ms_fwd,
+                                 "err_max": float(d.max()),
+                                 "err_rms": float(torch.sqrt((d * d).mean())),
+                                 "api_io_GBps": api_bytes / ((ms - ms_copy) * 1e-3) / 1e9,
+                                 "hbm_frac_api_io": api_bytes / ((ms - ms_copy) * 1e-3) / 1e9 /
+                                 HBM_PEAK_GBS}
             outs["f64_dropin"] = r.to(torch.int64)
 
     # bit-exact agreement of the rounded products across forms

@@ -52,6 +52,7 @@ def _declare(L):
     L.orc_cpu_ifft_f64.argtypes = [_I, _P, _P]
     L.orc_cpu_ffnt_mult_f64.argtypes = [_I, _P, _P, _P]
     L.orc_cpu_ffnt_mult_batch_f64.argtypes = [_I, _I, _P, _P, _P, _I]
+    L.orc_cpu_fft_mult_batch_f64.argtypes = [_I, _I, _P, _P, _P, _I]
     for s, T in (("f64", _D), ("f32", _F)):
         getattr(L, "orc_c2c_forward_" + s).argtypes = [_P, _I, _I, _P, _I]
         getattr(L, "orc_c2c_inverse_" + s).argtypes = [_P, _I, _I, _P, _I, _I, T, T]
@@ -131,6 +132,18 @@ def cpu_ifft(data, inv_root):
     assert data.dtype == np.complex128 and len(data) == 2 * n
     lib().orc_cpu_ifft_f64(n, _ptr(inv_root), _ptr(data))
     return data
+
+
+def cpu_fft_mult(a, b, threads=1):
+    """The reference's CPU plain-method product (example/test_fft_cpu.cu:65-142)
+    for rows of a, b (int64, shape [count, n]): linear products, [count, 2n]."""
+    a = np.ascontiguousarray(a, np.int64)
+    b = np.ascontiguousarray(b, np.int64)
+    if a.ndim == 1:
+        a, b = a[None], b[None]
+    out = np.zeros((a.shape[0], 2 * a.shape[1]), np.int64)
+    lib().orc_cpu_fft_mult_batch_f64(a.shape[1], a.shape[0], _ptr(a), _ptr(b), _ptr(out), threads)
+    return out
 
 
 def cpu_ffnt_mult(a, b, threads=1):

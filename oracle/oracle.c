@@ -337,6 +337,70 @@ void orc_cpu_ffnt_mult_batch_f64(int N, int count, const int64_t *a, const int64
     for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
 }
 
+/* The reference's CPU plain-method product (example/test_fft_cpu.cu:65-142):
+ * a, b of n coefficients zero-padded to 2n, FFT<double>(n)::fft of both,
+ * pointwise product, ifft, round the real parts: the linear product, 2n
+ * outputs (the last one 0).  `root` and `inv` are FFT(n).root_tables and
+ * inverse_root_tables (fft_cpu.cu:93-129); work holds 2 x 2n complex. */
+static void cpu_fft_mult_one(int n, const double *root, const double *inv, const int64_t *a,
+                             const int64_t *b, int64_t *out, c_f64 *work)
+{
+    c_f64 *x = work, *y = work + 2 * n;
+    for (int i = 0; i < 2 * n; i++) {
+        x[i] = mk_f64(i < n ? (double)a[i] : 0.0, 0.0);
+        y[i] = mk_f64(i < n ? (double)b[i] : 0.0, 0.0);
+    }
+    orc_cpu_fft_f64(n, root, (double *)x);
+    orc_cpu_fft_f64(n, root, (double *)y);
+    for (int i = 0; i < 2 * n; i++) x[i] = mul_f64(x[i], y[i]);
+    orc_cpu_ifft_f64(n, inv, (double *)x);
+    for (int i = 0; i < 2 * n; i++) out[i] = (int64_t)round(x[i].re);
+}
+
+typedef struct {
+    int n, begin, end;
+    const double *root, *inv;
+    const int64_t *a, *b;
+    int64_t *out;
+} fft_mult_job;
+
+static void *fft_mult_worker(void *p)
+{
+    fft_mult_job *j = (fft_mult_job *)p;
+    c_f64 *work = (c_f64 *)malloc(sizeof(c_f64) * 4 * (size_t)j->n);
+    for (int i = j->begin; i < j->end; i++)
+        cpu_fft_mult_one(j->n, j->root, j->inv, j->a + (size_t)i * j->n, j->b + (size_t)i * j->n,
+                         j->out + (size_t)i * 2 * j->n, work);
+    free(work);
+    return NULL;
+}
+
+void orc_cpu_fft_mult_batch_f64(int n, int count, const int64_t *a, const int64_t *b,
+                                int64_t *out, int threads)
+{
+    /* tables once per call, as the reference builds FFT<T>(n) once for a batch */
+    double *root = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+    double *inv = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+    double *rev = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+    double *inv_rev = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+    orc_fft_tables_f64(n, root, inv, rev, inv_rev);
+    if (threads < 1) threads = 1;
+    if (threads > count) threads = count;
+    if (threads > 256) threads = 256;
+    pthread_t th[256];
+    fft_mult_job jobs[256];
+    for (int t = 0; t < threads; t++) {
+        jobs[t].n = n;
+        jobs[t].begin = (int)((long long)count * t / threads);
+        jobs[t].end = (int)((long long)count * (t + 1) / threads);
+        jobs[t].root = root; jobs[t].inv = inv;
+        jobs[t].a = a; jobs[t].b = b; jobs[t].out = out;
+        pthread_create(&th[t], NULL, fft_mult_worker, &jobs[t]);
+    }
+    for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
+    free(root); free(inv); free(rev); free(inv_rev);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Observable semantics of the GPU entry points (fft.cu)                    */
 /* ------------------------------------------------------------------------ */

@@ -58,6 +58,12 @@ void orc_cpu_ffnt_mult_f64(int N, const int64_t *a, const int64_t *b, int64_t *o
 /* Same, for `count` independent products (stride N), on `threads` threads. */
 void orc_cpu_ffnt_mult_batch_f64(int N, int count, const int64_t *a, const int64_t *b,
                                  int64_t *out, int threads);
+/* The plain-method CPU product of example/test_fft_cpu.cu:65-142 (FFT<double>(n)
+ * fft of both zero-padded operands, product, ifft, round): `count` linear
+ * products of n-coefficient operands (stride n), 2n outputs each (stride 2n),
+ * on `threads` threads. */
+void orc_cpu_fft_mult_batch_f64(int n, int count, const int64_t *a, const int64_t *b,
+                                int64_t *out, int threads);
 
 /* ---- observable semantics of the GPU entry points ----
  * f64 and f32 variants; for f32 the arithmetic is done in float, as on

@@ -118,6 +118,21 @@ def test_r2r_matches_c2c_on_real_data(n_power):
     assert np.array_equal(r, c.real)
 
 
+@pytest.mark.parametrize("n", [1, 4, 64, 1024])
+def test_cpu_fft_mult_is_linear_product(n):
+    """The reference's CPU plain-method product (FFT::fft / ifft of the
+    zero-padded operands) equals numpy's convolution, batch-threaded or not."""
+    rng = np.random.default_rng(60 + n)
+    a = rng.integers(-1024, 1024, (3, n))
+    b = rng.integers(-1024, 1024, (3, n))
+    for threads in (1, 2):
+        got = O.cpu_fft_mult(a, b, threads=threads)
+        for i in range(3):
+            ref = np.zeros(2 * n, np.int64)
+            ref[:2 * n - 1] = np.convolve(a[i], b[i])
+            assert np.array_equal(got[i], ref)
+
+
 @pytest.mark.parametrize("n_power", [1, 3, 8])
 @pytest.mark.parametrize("mult", [0, 1])
 def test_r2r_inverse_general_temp(n_power, mult):
