@@ -43,6 +43,8 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=10.0)
     p.add_argument("--rb-f64", type=int, default=0, help="radix bits of the FP64 kernel (0: tune)")
     p.add_argument("--rb-f32", type=int, default=0, help="radix bits of the FP32 kernel (0: tune)")
+    p.add_argument("--e-f32", type=int, default=0,
+                   help="log2 points per thread of the FP32 kernel, 5 or 4 (0: tune)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--surface", action="store_true",
                    help="time the drop-in entry points on the reference's benchmark axes "
@@ -693,10 +695,24 @@ def main():
             plan.a_center_log2 = args.a_center
             out = torch.empty(batch * N, dtype=torch.int64, device=dev)
             err = torch.empty(2 * batch, dtype=torch.float64, device=dev)
-            tune = tune_radix(plan, [args.rb_f32] if args.rb_f32 else [3, 4, 5],
-                              lambda: plan.multiply_i32(a32, b32, out, batch, stream=sid),
-                              stream, args.steps)
-            plan.radix_bits = min(tune, key=tune.get)
+            tune, cands = {}, []
+            for e in ([args.e_f32] if args.e_f32 else [5, 4]):
+                plan.radix_bits = 4  # exists at every points-per-thread setting
+                plan.thread_points = e
+                for rb in ([args.rb_f32] if args.rb_f32 else [3, 4, 5]):
+                    try:
+                        plan.radix_bits = rb
+                    except ValueError:  # no kernel for this radix at these points per thread
+                        continue
+                    t = tune_radix(plan, [rb], lambda: plan.multiply_i32(a32, b32, out, batch,
+                                                                         stream=sid),
+                                   stream, args.steps)[rb]
+                    tune["e%d_rb%d" % (e, rb)] = t
+                    cands.append((t, e, rb))
+            _, best_e, best_rb = min(cands)
+            plan.radix_bits = 4
+            plan.thread_points = best_e
+            plan.radix_bits = best_rb
             ms = timed(lambda: plan.multiply_i32(a32, b32, out, batch, stream=sid), stream,
                        args.steps, args.warmup)
             ms_t = timed(lambda: plan.multiply_i32(a32, b32, out, batch, stream=sid,
@@ -707,7 +723,8 @@ def main():
             torch.cuda.synchronize()
             e = err.view(batch, 2)
             res["f32split"] = {"ms": ms, "ms_transform": ms_t, "ms_combine": ms_c,
-                               "radix_bits": plan.radix_bits, "tune_ms": tune,
+                               "radix_bits": plan.radix_bits,
+                               "thread_points_log2": plan.thread_points, "tune_ms": tune,
                                "limbs": "a%s:%dx%d-bit%s" % (
                                    " - 2^%d" % args.a_center if args.a_center else "", ka, wa,
                                    " + 2^%d (J b) limb" % args.a_center if args.a_center else ""),

@@ -31,6 +31,7 @@ struct gpufft_polymul_plan_s {
     void* jspec;     // FP32 split with ca: forward FFNT spectrum of J (complex float, L)
     int S, SB;       // sub-problem split
     int rb;          // radix bits per LDS step of the product kernel
+    int e;           // log2 points per thread of the product kernel (2^(S-e) threads)
     int cmb;         // FP64, SB > 0: 1 = top stages in the product kernel, 0 = separate pass (default)
     void* root;      // forward FFNT root table (L entries)
     void* iroot;
@@ -51,6 +52,7 @@ constexpr int MAX_SB = 3;
 #ifndef GFA_E64
 #define GFA_E64 4
 #endif
+
 
 
 template <typename T> using ProdKernel = void (*)(ProdArgs<T>);
@@ -75,10 +77,19 @@ __global__ void zero_ints(int* p, int n)
     if (i < n) p[i] = 0;
 }
 
-template <typename T, int PL, int RB> ProdKernel<T> prod_kernel_rb(int S, int SB, int cmb)
+// Whether the product kernel exists for a sub-problem of 2^S points, 2^E
+// points per thread and radix-2^RB steps: the thread's groups of V loaded
+// positions must hold the 2^(S mod RB) partners of the register top step.
+template <typename T, int S, int E, int PL, int RB> constexpr bool prod_ok()
 {
-    // E = 4 (double) / 5 (float): 512 threads per 128 KiB tile
-    constexpr int E = sizeof(T) == 8 ? GFA_E64 : 5;
+    constexpr int HB = S % RB;
+    constexpr int V = in_vec<T, PL, E, HB>();
+    return RB <= E && S > E && ((1 << E) / V) / (1 << HB) >= 1;
+}
+
+// 2^E points per thread: 2^(S-E) threads per workgroup
+template <typename T, int PL, int RB, int E> ProdKernel<T> prod_kernel_rb(int S, int SB, int cmb)
+{
     constexpr int SMAX = sizeof(T) == 8 ? F64_MAX_S : F32_MAX_S;
     if constexpr (sizeof(T) == 8 && PL == PL_FOLD) {
         if (cmb && S == SMAX) {
@@ -87,8 +98,9 @@ template <typename T, int PL, int RB> ProdKernel<T> prod_kernel_rb(int S, int SB
             if (SB == 3) return &product_sub_kernel<T, SMAX, 3, E, PL, RB, true>;
         }
     }
-#define GFA_PK(SS, BB) \
-    if (S == SS && SB == BB) return &product_sub_kernel<T, SS, BB, E, PL, RB>;
+#define GFA_PK(SS, BB)                                                 \
+    if constexpr (prod_ok<T, SS, E, PL, RB>())                         \
+        if (S == SS && SB == BB) return &product_sub_kernel<T, SS, BB, E, PL, RB>;
     if (SB > 0) {
         GFA_PK(SMAX, 1)
         GFA_PK(SMAX, 2)
@@ -110,18 +122,28 @@ template <typename T, int PL, int RB> ProdKernel<T> prod_kernel_rb(int S, int SB
     return nullptr;
 }
 
-// radix bits per LDS step: 3 or 4 (double), 3..5 (float); chosen per plan
-template <typename T, int PL> ProdKernel<T> prod_kernel(int S, int SB, int rb, int cmb)
+// radix bits per LDS step: 3 or 4 (double), 3..5 (float); points per thread
+// 2^e: 16 (double), 32 or 16 (float); both chosen per plan
+template <typename T, int PL, int E> ProdKernel<T> prod_kernel_e(int S, int SB, int rb, int cmb)
 {
     switch (rb) {
-        case 3: return prod_kernel_rb<T, PL, 3>(S, SB, cmb);
-        case 4:
-            if constexpr (sizeof(T) == 4 || GFA_E64 >= 4) return prod_kernel_rb<T, PL, 4>(S, SB, cmb);
-            return nullptr;
+        case 3: return prod_kernel_rb<T, PL, 3, E>(S, SB, cmb);
+        case 4: return prod_kernel_rb<T, PL, 4, E>(S, SB, cmb);
         case 5:
-            if constexpr (sizeof(T) == 4) return prod_kernel_rb<T, PL, 5>(S, SB, cmb);
+            if constexpr (sizeof(T) == 4 && E >= 5) return prod_kernel_rb<T, PL, 5, E>(S, SB, cmb);
             return nullptr;
         default: return nullptr;
+    }
+}
+
+template <typename T, int PL> ProdKernel<T> prod_kernel(int S, int SB, int rb, int cmb, int e)
+{
+    if constexpr (sizeof(T) == 8) {
+        return e == GFA_E64 ? prod_kernel_e<T, PL, GFA_E64>(S, SB, rb, cmb) : nullptr;
+    } else {
+        if (e == 5) return prod_kernel_e<T, PL, 5>(S, SB, rb, cmb);
+        if (e == 4) return prod_kernel_e<T, PL, 4>(S, SB, rb, cmb);
+        return nullptr;
     }
 }
 
@@ -195,10 +217,9 @@ int launch_products(const gpufft_polymul_plan_s* pl, const void* a, const void* 
     pa.oa = pl->ka > 1 ? offset(pl->wa, pl->ka) : 0;
     pa.ob = pl->kb > 1 ? offset(pl->wb, pl->kb) : 0;
     pa.arrive = pl->arrive;
-    ProdKernel<T> k = prod_kernel<T, PL>(pl->S, pl->SB, pl->rb, cmb);
+    ProdKernel<T> k = prod_kernel<T, PL>(pl->S, pl->SB, pl->rb, cmb, pl->e);
     if (!k) return fail(GPUFFT_ERR_UNSUPPORTED, std::string(who) + ": no kernel for this size");
-    constexpr int E = sizeof(T) == 8 ? GFA_E64 : 5;
-    const int threads = (1 << pl->S) >> E;
+    const int threads = (1 << pl->S) >> pl->e;
     const long long groups = ((long long)batch + 7) / 8;
     const long long blocks = groups * 8 * (1LL << pl->SB);
     hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(threads), 0, st, pa);
@@ -335,6 +356,10 @@ int gpufft_polymul_plan_create(gpufft_polymul_plan* out, int ring_power, int pre
     pl->S = std::min(n, smax);
     pl->SB = n - pl->S;
     pl->rb = 4;
+    // FP32: 1024 threads of 16 points per full 2^14-point tile (measured
+    // faster than 512 threads of 32 at N = 2^16: 0.38 against 0.45 ms per
+    // batch of 256), 512 threads of 32 points below the full tile
+    pl->e = precision == GPUFFT_PREC_F64 ? GFA_E64 : (pl->S == F32_MAX_S ? 4 : 5);
     pl->cmb = 0;  // separate pass: measured faster at N = 2^16, batch 256
     hipStream_t st = (hipStream_t)stream;
     int s = precision == GPUFFT_PREC_F64 ? make_tables<double>(pl, st) : make_tables<float>(pl, st);
@@ -472,8 +497,31 @@ int gpufft_polymul_plan_set_radix_bits(gpufft_polymul_plan plan, int radix_bits)
     const int hi = plan->precision == GPUFFT_PREC_F64 ? 4 : 5;
     if (radix_bits < 3 || radix_bits > hi)
         return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": radix_bits out of range");
+    if (plan->precision == GPUFFT_PREC_F32_SPLIT &&
+        !prod_kernel<float, PL_FOLD_LIMB>(plan->S, plan->SB, radix_bits, 0, plan->e))
+        return fail(GPUFFT_ERR_UNSUPPORTED,
+                    std::string(who) + ": no kernel for this radix at the plan's points per thread");
     plan->rb = radix_bits;
     return ok();
+}
+
+int gpufft_polymul_plan_set_thread_points(gpufft_polymul_plan plan, int log2_points)
+{
+    const char* who = "gpufft_polymul_plan_set_thread_points";
+    if (!plan) return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": plan is NULL");
+    const bool okv = plan->precision == GPUFFT_PREC_F64 ? log2_points == GFA_E64
+                                                        : (log2_points == 4 || log2_points == 5);
+    if (!okv) return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": unsupported points per thread");
+    if (!prod_kernel<float, PL_FOLD_LIMB>(plan->S, plan->SB, plan->rb, 0, log2_points) &&
+        plan->precision == GPUFFT_PREC_F32_SPLIT)
+        return fail(GPUFFT_ERR_UNSUPPORTED, std::string(who) + ": no kernel for this size and radix");
+    plan->e = log2_points;
+    return ok();
+}
+
+int gpufft_polymul_plan_get_thread_points(gpufft_polymul_plan plan)
+{
+    return plan ? plan->e : -1;
 }
 
 int gpufft_polymul_plan_get_radix_bits(gpufft_polymul_plan plan)
@@ -564,6 +612,7 @@ int gpufft_polymul_f64_noalloc(const double* a, const double* b, double* out, in
     pl.S = std::min(n, F64_MAX_S);
     pl.SB = n - pl.S;
     pl.rb = 4;
+    pl.e = GFA_E64;
     pl.cmb = 0;
     pl.root = const_cast<void*>(root_table);
     pl.iroot = const_cast<void*>(inv_root_table);

@@ -94,6 +94,8 @@ _SIGS = {
     "gpufft_batched_execute": [_P, _P, _P, _P, _P, _P],
     "gpufft_batched_graph_launch": [_P, _P, _P, _P, _P, _P],
     "gpufft_polymul_plan_get_radix_bits": [_P],
+    "gpufft_polymul_plan_set_thread_points": [_P, _I],
+    "gpufft_polymul_plan_get_thread_points": [_P],
     "gpufft_polymul_plan_set_combine": [_P, _I],
     "gpufft_polymul_plan_get_combine": [_P],
     "gpufft_polymul_plan_set_a_center": [_P, _I],
@@ -275,6 +277,17 @@ class PolymulPlan:
     @radix_bits.setter
     def radix_bits(self, rb):
         check(_lib.gpufft_polymul_plan_set_radix_bits(self.handle, rb))
+
+    @property
+    def thread_points(self):
+        """log2 of the points per thread of the product kernel: 4 for FP64
+        (512 threads per 2^13-point sub-problem); 5 or 4 for the FP32 split
+        (512 or 1024 threads per 2^14-point sub-problem)."""
+        return _lib.gpufft_polymul_plan_get_thread_points(self.handle)
+
+    @thread_points.setter
+    def thread_points(self, e):
+        check(_lib.gpufft_polymul_plan_set_thread_points(self.handle, int(e)))
 
     @property
     def combine_in_kernel(self):

@@ -195,6 +195,14 @@ size_t gpufft_polymul_plan_scratch_bytes(gpufft_polymul_plan plan);
  * sets it per ring size and device. */
 int gpufft_polymul_plan_set_radix_bits(gpufft_polymul_plan plan, int radix_bits);
 int gpufft_polymul_plan_get_radix_bits(gpufft_polymul_plan plan);
+/* Points per thread of the product kernel, 2^log2_points: 4 for
+ * GPUFFT_PREC_F64 (512 threads per 2^13-point sub-problem); 5 (default, 512
+ * threads) or 4 (1024 threads) per 2^14-point sub-problem for
+ * GPUFFT_PREC_F32_SPLIT.  Results do not depend on it beyond rounding.
+ * Fails with GPUFFT_ERR_UNSUPPORTED where no kernel exists for the plan's
+ * size and radix (16 points per thread allow radix bits 3 and 4 only). */
+int gpufft_polymul_plan_set_thread_points(gpufft_polymul_plan plan, int log2_points);
+int gpufft_polymul_plan_get_thread_points(gpufft_polymul_plan plan);
 /* GPUFFT_PREC_F64 plans whose product spans several workgroups (ring_power
  * >= 15): in_kernel = 1 runs the last inverse stages inside the product
  * kernel, in the workgroup that finishes a product last (one launch); 0

@@ -141,10 +141,15 @@ SPLITS = {
 BEYOND_BOUND = {(17, "a5x5"), (17, "a7x3_b4x2")}
 
 
-@pytest.mark.parametrize("rb", [3, 4, 5])
+# (log2 points per thread, radix bits): 512 threads per 2^14-point
+# sub-problem with radix 8, 16 or 32 steps, or 1024 threads with radix 8 or 16
+THREAD_RADIX = [(5, 3), (5, 4), (5, 5), (4, 3), (4, 4)]
+
+
+@pytest.mark.parametrize("e,rb", THREAD_RADIX)
 @pytest.mark.parametrize("split", sorted(SPLITS))
 @pytest.mark.parametrize("ring_power", [8, 12, 13, 14, 15, 16, 17])
-def test_polymul_f32_split_exact(ring_power, split, rb):
+def test_polymul_f32_split_exact(ring_power, split, e, rb):
     G = _g()
     import torch
     N = 1 << ring_power
@@ -152,7 +157,10 @@ def test_polymul_f32_split_exact(ring_power, split, rb):
     a, b = _operands(N, batch, SEED_BASE + 33 + ring_power)
     wa, ka, wb, kb = SPLITS[split][:4]
     plan = G.PolymulPlan(ring_power, G.PREC_F32_SPLIT, batch, wa, ka, wb, kb)
+    plan.radix_bits = min(rb, 4)
+    plan.thread_points = e
     plan.radix_bits = rb
+    assert (plan.thread_points, plan.radix_bits) == (e, rb)
     if len(SPLITS[split]) > 4:
         plan.a_center_log2 = SPLITS[split][4]
         assert plan.a_center_log2 == SPLITS[split][4]
