@@ -173,7 +173,7 @@ __global__ __launch_bounds__((1 << S) >> E) void inv_local_kernel(OneArgs<T> o)
     }
 }
 
-// Inverse, second launch: the top SB Gentleman-Sande stages (bits S .. n-1)
+// Inverse, second launch (SB = 2..4): the top SB Gentleman-Sande stages (bits S .. n-1)
 // of two adjacent columns c, c+1 per thread from the 2^SB rows j 2^S + c of
 // the first launch's result, twiddle tab[base_b + (j0 >> (i+1))] (uniform:
 // the column bits are below the stage bit), then the scale and the output:
@@ -186,7 +186,7 @@ template <typename T, int SB, int SK>
 __global__ __launch_bounds__(256) void inv_top_kernel(OneArgs<T> o, int S)
 {
     constexpr int Q = 1 << SB;
-    constexpr int UTN = SK == ST_UNFOLD ? (1 << (14 + 3 - 7)) + 128 : 1;
+    constexpr int UTN = SK == ST_UNFOLD ? (1 << (17 - 7)) + 128 : 1;  // n <= 17
     __shared__ cplx<T> ut[UTN];
     const int tid = threadIdx.x;
     const int n = o.n;

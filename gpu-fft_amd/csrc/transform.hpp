@@ -186,11 +186,18 @@ int run_transform(int mode, int n, int batch, int nega, const cplx<T>* tab,
 // the in-sub-problem stages (in place in the source) and a streaming launch
 // of the top SB stages with the output transform.
 template <typename T> struct OneCfg;
+// The forward's workgroups read the whole transform, so it takes the largest
+// tile (one workgroup per CU); the inverse's first launch reads only its own
+// sub-problem and takes half of it, so that two workgroups share a CU
+// (measured at N = 2^16, batch 256, FP64: inverse 0.146 -> 0.135 ms; the
+// forward at the smaller tile went 0.150 -> 0.161 ms).
 template <> struct OneCfg<double> {
-    static constexpr int S = 13, E = 4, RB = 4;  // 512 threads, 16 points each
+    static constexpr int S = 13, E = 4, RB = 4;   // forward: 512 threads, 16 points each
+    static constexpr int SI = 12, EI = 4;         // inverse: 256 threads, 16 points each
 };
 template <> struct OneCfg<float> {
-    static constexpr int S = 14, E = 5, RB = 4;  // 512 threads, 32 points each
+    static constexpr int S = 14, E = 5, RB = 4;   // forward: 512 threads, 32 points each
+    static constexpr int SI = 13, EI = 5;         // inverse: 256 threads, 32 points each
 };
 
 template <typename T> using OneFn = void (*)(OneArgs<T>);
@@ -204,7 +211,7 @@ template <typename T, int SB> OneFn<T> select_forward_sb(int lk)
     return nullptr;
 }
 
-template <typename T, int SB> TopFn<T> select_top_sb(int sk)
+template <typename T, int SB> TopFn<T> select_top_sb(int sk)  // SB = 2..4
 {
     if (sk == ST_REAL) return &inv_top_kernel<T, SB, ST_REAL>;
     if (sk == ST_UNFOLD) return &inv_top_kernel<T, SB, ST_UNFOLD>;
@@ -247,19 +254,21 @@ bool run_oneshot(int mode, int n, int batch, int nega, const cplx<T>* tab,
         hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3((1 << S) >> E), 0, stream, a);
     } else if (mode == MODE_STD_INV && (io.lk == LD_CPLX || io.lk == LD_MUL) &&
                (io.sk == ST_REAL || io.sk == ST_UNFOLD)) {
-        const long long blocks1 = (long long)batch << SB;
-        const long long blocks2 = (((long long)batch << (S - 1)) + 255) / 256;
+        constexpr int SI = OneCfg<T>::SI, EI = OneCfg<T>::EI;
+        const int SBI = n - SI;  // 2..4
+        const long long blocks1 = (long long)batch << SBI;
+        const long long blocks2 = (((long long)batch << (SI - 1)) + 255) / 256;
         if (blocks1 > 0x7fffffffLL || blocks2 > 0x7fffffffLL) {
             *status = fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": batch too large");
             return true;
         }
-        OneFn<T> f1 = io.lk == LD_MUL ? &inv_local_kernel<T, S, E, RB, LD_MUL>
-                                      : &inv_local_kernel<T, S, E, RB, LD_CPLX>;
-        hipLaunchKernelGGL(f1, dim3((unsigned)blocks1), dim3((1 << S) >> E), 0, stream, a);
-        TopFn<T> f2 = SB == 1 ? select_top_sb<T, 1>(io.sk)
-                    : SB == 2 ? select_top_sb<T, 2>(io.sk)
-                              : select_top_sb<T, 3>(io.sk);
-        hipLaunchKernelGGL(f2, dim3((unsigned)blocks2), dim3(256), 0, stream, a, S);
+        OneFn<T> f1 = io.lk == LD_MUL ? &inv_local_kernel<T, SI, EI, RB, LD_MUL>
+                                      : &inv_local_kernel<T, SI, EI, RB, LD_CPLX>;
+        hipLaunchKernelGGL(f1, dim3((unsigned)blocks1), dim3((1 << SI) >> EI), 0, stream, a);
+        TopFn<T> f2 = SBI == 2 ? select_top_sb<T, 2>(io.sk)
+                    : SBI == 3 ? select_top_sb<T, 3>(io.sk)
+                               : select_top_sb<T, 4>(io.sk);
+        hipLaunchKernelGGL(f2, dim3((unsigned)blocks2), dim3(256), 0, stream, a, SI);
     } else {
         return false;
     }
