@@ -851,7 +851,14 @@ def main():
                    "form": best, "ring_size": N, "batch": batch},
         "roofline": {"bound": "hbm", "kernel": "%s/%s" % (best, kname), "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic},
+                     "traffic": traffic,
+                     # the kernel's algorithmic bytes per launch, the HBM bytes its
+                     # counters read per launch over them, and the whole step
+                     # (every launch of the form) against the same peak
+                     "kernel_alg_bytes": kbytes,
+                     "traffic_ratio": (traffic / kbytes) if traffic else None,
+                     "step_achieved": r_best["alg_GBps"],
+                     "step_frac": r_best["alg_GBps"] / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
         "cpu_baseline_parallel": cpu_par,
         "forms": res,
