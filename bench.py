@@ -218,12 +218,13 @@ def batched_uneven(args):
     """Batched uneven form: `count` products, ring sizes 2^e with e uniform in
     {10..17}, even index linear (C_C, zero-padded cyclic), odd negacyclic,
     coefficients uniform in [-2^8, 2^8), seeded.  One persistent launch and
-    its combine launch replayed as a HIP graph (FP32 split, a and b in 2 limbs
-    of 5 bits; radix, grid, combine placement and LDS tile tuned), against
-    one launch per (kind, ring size) group in FP64
+    its combine launch replayed as a HIP graph, in both precisions: FP32
+    split (a and b in 2 limbs of 5 bits; radix, grid, combine placement and
+    LDS tile tuned) and FP64 (one limb each; radix, grid and combine
+    placement tuned), against one launch per (kind, ring size) group in FP64
     (fused negacyclic plans; drop-in GPU_FFT C_C pairs for the linear
-    products).  Rounded products of the persistent launch are checked
-    against the FP64 launches."""
+    products).  Rounded products of the persistent launches are checked
+    against each other and against the per-size FP64 launches."""
     import numpy as np
     import torch
 
@@ -253,6 +254,22 @@ def batched_uneven(args):
         ms = A.time_ms(lambda: plan.graph_launch(da, db, out, stream=sid), s, reps=max(3, args.steps // 4))
     s.synchronize()
     got = out.cpu().numpy()
+    max_err32 = float(err.max())
+    plan.close()
+    # FP64 persistent plan, same list
+    plan64 = G.BatchedPlan(descs, stream=sid, precision=G.PREC_F64)
+    out64 = torch.zeros(n_out, dtype=torch.int64, device=dev)
+    with torch.cuda.stream(s):
+        cand64, _ = A.tune_batched(plan64, lambda: plan64.execute(da, db, out64, stream=sid), s,
+                                   key=A.signature("batched_uneven_f64", count), use_db=False)
+        err.zero_()
+        plan64.execute(da, db, out64, err, stream=sid)
+        ms64 = A.time_ms(lambda: plan64.graph_launch(da, db, out64, stream=sid), s,
+                         reps=max(3, args.steps // 4))
+    s.synchronize()
+    max_err64 = float(err.max())
+    same64 = bool(np.array_equal(out64.cpu().numpy(), got))
+    plan64.close()
     alg = sum(2 * (1 << int(d.ring_power)) * 4 + G.output_length(d.kind, d.ring_power) * 8
               for d in descs)
     # FP64: one launch (pair) per (kind, ring size) group, same products
@@ -335,13 +352,21 @@ def batched_uneven(args):
     print(json.dumps({**cpu, "workload": "batched uneven products", "count": count,
                       "ring_powers": "10..17", "kinds": "half linear (C_C), half negacyclic",
                       "coefficients": "[-2^8, 2^8)", "fp32_split": "a, b: 2 limbs of 5 bits",
-                      "tuned": list(cand),
                       "tuned_fields": "radix bits, grid, combine in kernel, log2 LDS tile",
-                      "best_eager_ms_per_tile": {str(k): v for k, v in sorted(best_per_tile.items())},
-                      "persistent_graph_ms": ms,
-                      "products_per_s": count / (ms * 1e-3), "alg_GBps": alg / (ms * 1e-3) / 1e9,
-                      "max_err": float(err.max()), "fp64_per_size_ms": t64,
-                      "size_groups": len(groups), "bit_exact_vs_fp64": bool(exact)}))
+                      "forms": {
+                          "f32split": {"persistent_graph_ms": ms, "tuned": list(cand),
+                                       "best_eager_ms_per_tile": {str(k): v for k, v in
+                                                                  sorted(best_per_tile.items())},
+                                       "max_err": max_err32,
+                                       "bit_exact_vs_fp64_per_size": bool(exact)},
+                          "f64": {"persistent_graph_ms": ms64, "tuned": list(cand64),
+                                  "max_err": max_err64, "bit_exact_vs_f32split": same64}},
+                      "best_form": "f64" if ms64 < ms else "f32split",
+                      "persistent_graph_ms": min(ms, ms64),
+                      "products_per_s": count / (min(ms, ms64) * 1e-3),
+                      "alg_GBps": alg / (min(ms, ms64) * 1e-3) / 1e9,
+                      "fp64_per_size_ms": t64, "size_groups": len(groups),
+                      "bit_exact_vs_fp64": bool(exact) and same64}))
 
 
 def _eval_mod(coeffs, xs, p=2147483647):

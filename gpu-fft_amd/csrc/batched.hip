@@ -1,6 +1,7 @@
-// Batched products of uneven ring sizes in one persistent launch (FP32 limb
-// split, int32 in / int64 out).  No counterpart in the reference, which runs
-// one GPU_FFT / GPU_FFNT call pair per ring size.
+// Batched products of uneven ring sizes in one persistent launch, int32 in /
+// int64 out, in one of two precisions: FP32 with a and b split into limbs, or
+// FP64 (one limb each).  No counterpart in the reference, which runs one
+// GPU_FFT / GPU_FFNT call pair per ring size.
 //
 // Both product kinds are FFNT products (fold + twist, L complex points;
 // reference GPU_FFNT, src/lib/fft.cu:1013-1384):
@@ -14,9 +15,9 @@
 //               transform length.
 //
 // Every product is cut into 2^SB sub-problems of 2^S points, S = min(log2 L,
-// T), as in fused.hpp; T is the plan's LDS tile, 14 (128 KiB, one workgroup
-// per CU) or 13 (64 KiB, two per CU).  The work list holds one item per
-// sub-problem.  The
+// T), as in fused.hpp; T is the plan's LDS tile: FP32 14 (128 KiB, one
+// workgroup per CU) or 13 (64 KiB, two per CU), FP64 13 (128 KiB).  The work
+// list holds one item per sub-problem.  The
 // list is split into 8 queues, one per XCD: the hardware deals the workgroups
 // of the persistent grid round-robin to the 8 XCDs, and workgroup b serves
 // queue b mod 8 first.  All sub-problems of a product sit in one queue, so
@@ -62,6 +63,7 @@ constexpr int KIND_NEGA = 0;
 constexpr int KIND_LINEAR = 1;
 constexpr int BT = 512;     // threads of the persistent workgroup
 constexpr int SMAX = 14;    // largest tile: 2^14 float complex = 128 KiB of LDS
+constexpr int SMAX64 = 13;  // FP64: 2^13 double complex = 128 KiB
 constexpr int SMIN = 5;
 constexpr int SBMAX = 4;
 constexpr int NQ = 8;       // work queues, one per XCD
@@ -74,15 +76,21 @@ struct BProd {
     int N;        // coefficients per operand
     int tabset;   // index of the table set for n
     long long a_off, b_off, out_off;
-    long long scratch_off;  // cplx<float> offset (SB > 0)
+    long long scratch_off;  // cplx<T> offset (SB > 0)
 };
 
+// FFNT tables of one L, cplx<float> or cplx<double> as the plan's precision
 struct BTables {
-    const cplx<float>* root;
-    const cplx<float>* iroot;
-    const cplx<float>* twist;
-    const cplx<float>* untwist;
+    const void* root;
+    const void* iroot;
+    const void* twist;
+    const void* untwist;
 };
+
+template <typename T> __device__ __forceinline__ const cplx<T>* tabp(const void* p)
+{
+    return static_cast<const cplx<T>*>(p);
+}
 
 // Order of the limb products of one product.  Limb products of equal
 // weight wa ia + wb jb form a group: they are summed in the frequency domain
@@ -113,12 +121,12 @@ struct BArgs {
     const int* a;
     const int* b;
     long long* out;
-    cplx<float>* scratch;
+    void* scratch;       // cplx<T> sub-results of split products
     const BSched* sched;
-    cplx<float>* stash;  // stash_slots x 2^SMAX per workgroup of the persistent grid
+    void* stash;         // cplx<T>: stash_slots x 2^SMAX per workgroup of the persistent grid
     int stash_slots;     // b-limb spectra 1 .. kb-1, then group partial sums
-    const cplx<float>* wc;
-    const cplx<float>* wci;
+    const void* wc;      // stage root constants, cplx<T>
+    const void* wci;
     int wa, ka, wb, kb;
     long long oa, ob;
     double* err;        // optional [products]
@@ -165,6 +173,33 @@ template <typename T> __device__ __forceinline__ gptr<T> to_global(T* p)
     return (gptr<T>)p;
 }
 
+// Digit `limb` of an operand coefficient as T (one limb: the coefficient
+// itself, exact in FP32 below 2^24 and in FP64 for every int32).
+template <typename T>
+__device__ __forceinline__ T item_digit(int v, int w, int limb, int k, int o)
+{
+    if (k == 1) return (T)v;
+    return (T)limb_digit(v, w, limb, k, o, 0);
+}
+
+// Twist factors of the 4 consecutive positions c0 .. c0+3 (16-byte loads).
+template <typename T>
+__device__ __forceinline__ void twist4(gptr<const cplx<T>> twist, int c0, cplx<T>* t)
+{
+    if constexpr (sizeof(T) == 4) {
+        typedef float f32x4 __attribute__((ext_vector_type(4)));
+        const f32x4 t0 = *(gptr<const f32x4>)(twist + c0);
+        const f32x4 t1 = *(gptr<const f32x4>)(twist + c0 + 2);
+        t[0] = mk<float>(t0[0], t0[1]);
+        t[1] = mk<float>(t0[2], t0[3]);
+        t[2] = mk<float>(t1[0], t1[1]);
+        t[3] = mk<float>(t1[2], t1[3]);
+    } else {
+#pragma unroll
+        for (int v = 0; v < 4; v++) t[v] = twist[c0 + v];
+    }
+}
+
 // Sub-problem q of one limb of one operand into LDS: for each of the
 // workgroup's 2^S positions c, row q of the top SB forward stages over the
 // folded inputs at j 2^S + c (Q = 2^SB partners), times the twist
@@ -172,10 +207,15 @@ template <typename T> __device__ __forceinline__ gptr<T> to_global(T* p)
 // half.  A lane takes 4 consecutive positions with one 16-byte load per
 // partner (operand offsets are multiples of 4 elements), U such groups per
 // round with all their loads issued together.
-template <int KIND, int S, int Q, int E>
-__device__ __forceinline__ void item_load(cplx<float>* lds, gptr<const cplx<float>> twist,
+// FP64 (TWL): C and the twist factors come from LDS (C written by the
+// item, twist[c] = tws[c >> 7] tws[NHI + (c & 127)]), which keeps the 2^SB
+// double coefficients and the twist loads out of the registers and the
+// vector memory path.
+template <typename T, int KIND, int S, int Q, int E, bool TWL = false>
+__device__ __forceinline__ void item_load(cplx<T>* lds, gptr<const cplx<T>> twist,
                                           gptr<const int> x, int L, int w_in, int limb_in,
-                                          int k_in, int o_in, const cplx<float>* C, int tid)
+                                          int k_in, int o_in, const cplx<T>* C, int tid,
+                                          const cplx<T>* tws = nullptr)
 {
     // the split parameters are uniform: scalar copies keep the digit
     // extraction free of per-lane branches
@@ -184,7 +224,6 @@ __device__ __forceinline__ void item_load(cplx<float>* lds, gptr<const cplx<floa
     const int k = __builtin_amdgcn_readfirstlane(k_in);
     const int o = __builtin_amdgcn_readfirstlane(o_in);
     typedef int i32x4 __attribute__((ext_vector_type(4)));
-    typedef float f32x4 __attribute__((ext_vector_type(4)));
     constexpr int NT = (1 << S) >> E;
     constexpr int NV = (1 << E) / 4;  // groups of 4 positions per lane
     constexpr int U0 = Q >= 4 ? 1 : (Q == 2 ? 2 : 4);
@@ -192,12 +231,11 @@ __device__ __forceinline__ void item_load(cplx<float>* lds, gptr<const cplx<floa
     if (tid >= NT) return;
     for (int i0 = 0; i0 < NV; i0 += U) {
         i32x4 xr[U][Q], xi[U][Q];
-        f32x4 tw[U][2];
+        cplx<T> tw[U][4];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int c0 = (tid + (i0 + u) * NT) * 4;
-            tw[u][0] = *(gptr<const f32x4>)(twist + c0);
-            tw[u][1] = *(gptr<const f32x4>)(twist + c0 + 2);
+            if constexpr (!TWL) twist4<T>(twist, c0, tw[u]);
 #pragma unroll
             for (int j = 0; j < Q; j++) {
                 const int P = (j << S) | c0;
@@ -210,12 +248,12 @@ __device__ __forceinline__ void item_load(cplx<float>* lds, gptr<const cplx<floa
             const int c0 = (tid + (i0 + u) * NT) * 4;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                float ax = 0.f, ay = 0.f;
+                T ax = T(0), ay = T(0);
 #pragma unroll
                 for (int j = 0; j < Q; j++) {
-                    const float re = limb_digit(xr[u][j][v], w, limb, k, o, 0);
+                    const T re = item_digit<T>(xr[u][j][v], w, limb, k, o);
                     if constexpr (KIND == KIND_NEGA) {
-                        const float im = limb_digit(xi[u][j][v], w, limb, k, o, 0);
+                        const T im = item_digit<T>(xi[u][j][v], w, limb, k, o);
                         ax = ax + C[j].x * re - C[j].y * im;
                         ay = ay + C[j].x * im + C[j].y * re;
                     } else {
@@ -223,29 +261,46 @@ __device__ __forceinline__ void item_load(cplx<float>* lds, gptr<const cplx<floa
                         ay = ay + C[j].y * re;
                     }
                 }
-                const f32x4 t2 = tw[u][v >> 1];
-                const cplx<float> t = (v & 1) ? mk<float>(t2[2], t2[3]) : mk<float>(t2[0], t2[1]);
-                lds[swz(c0 + v)] = t * mk<float>(ax, ay);
+                if constexpr (TWL)
+                    lds[swz(c0 + v)] = twist_of<T, S>(tws, c0 + v) * mk<T>(ax, ay);
+                else
+                    lds[swz(c0 + v)] = tw[u][v] * mk<T>(ax, ay);
             }
         }
     }
 }
 
 // Streaming load and store of one element through a global pointer.
-__device__ __forceinline__ cplx<float> ldg_nt(gptr<const cplx<float>> p)
+template <typename T> __device__ __forceinline__ cplx<T> ldg_nt(gptr<const cplx<T>> p)
 {
-    typedef float vec2 __attribute__((ext_vector_type(2)));
+    typedef T vec2 __attribute__((ext_vector_type(2)));
     const vec2 w = __builtin_nontemporal_load((gptr<const vec2>)p);
-    return mk<float>(w.x, w.y);
+    return mk<T>(w.x, w.y);
 }
 
-__device__ __forceinline__ void stg_nt(gptr<cplx<float>> p, cplx<float> v)
+template <typename T> __device__ __forceinline__ void stg_nt(gptr<cplx<T>> p, cplx<T> v)
 {
-    typedef float vec2 __attribute__((ext_vector_type(2)));
+    typedef T vec2 __attribute__((ext_vector_type(2)));
     vec2 w;
     w.x = v.x;
     w.y = v.y;
     __builtin_nontemporal_store(w, (gptr<vec2>)p);
+}
+
+// Rounding to the nearest integer (ties to even) and the distance to it.
+template <typename T> __device__ __forceinline__ T rnd(T x)
+{
+    if constexpr (sizeof(T) == 4)
+        return rintf(x);
+    else
+        return rint(x);
+}
+
+template <typename T> __device__ __forceinline__ float rdist(cplx<T> z, T r0, T r1)
+{
+    const T d0 = z.x - r0, d1 = z.y - r1;
+    const T a0 = d0 < T(0) ? -d0 : d0, a1 = d1 < T(0) ? -d1 : d1;
+    return (float)(a0 > a1 ? a0 : a1);
 }
 
 // The top SB inverse stages of position c < 2^S of a split product (S the
@@ -254,51 +309,60 @@ __device__ __forceinline__ void stg_nt(gptr<cplx<float>> p, cplx<float> v)
 // recombination; writes the unfolded outputs out[P], out[P + L] for
 // P = j 2^S + c.  Returns the largest distance of a group output to its
 // rounding.
-template <int SB>
+template <typename T, int SB>
 __device__ __forceinline__ float combine_pos(const BArgs& ba, const BProd& pr, const BTables& tb,
                                              int c)
 {
     constexpr int Q = 1 << SB;
+    constexpr bool ONE = sizeof(T) == 8;  // one group: outputs stored as they are rounded
     const int S = pr.S;
     const int L = 1 << pr.n;
-    const float scale = 1.0f / (float)L;
-    const int ngrp = ba.sched->G;
-    long long acc[Q][2];
+    const T scale = T(1) / (T)L;
+    const int ngrp = ONE ? 1 : ba.sched->G;
+    const gptr<long long> o = to_global(ba.out + pr.out_off);
+    long long acc[ONE ? 1 : Q][2];
 #pragma unroll
-    for (int j = 0; j < Q; j++) acc[j][0] = acc[j][1] = 0;
+    for (int j = 0; j < (ONE ? 1 : Q); j++) acc[j][0] = acc[j][1] = 0;
     float em = 0.f;
-    const gptr<const cplx<float>> untwist = to_global(tb.untwist);
+    const gptr<const cplx<T>> untwist = to_global(tabp<T>(tb.untwist));
+    const cplx<T>* iroot = tabp<T>(tb.iroot);
     for (int l = 0; l < ngrp; l++) {
-        const gptr<const cplx<float>> src = to_global(ba.scratch + pr.scratch_off + (long long)l * L);
-        cplx<float> y[Q];
+        const gptr<const cplx<T>> src =
+            to_global(static_cast<const cplx<T>*>(ba.scratch) + pr.scratch_off + (long long)l * L);
+        cplx<T> y[Q];
 #pragma unroll
-        for (int j = 0; j < Q; j++) y[j] = ldg_nt(src + (j << S) + c);
+        for (int j = 0; j < Q; j++) y[j] = ldg_nt<T>(src + (j << S) + c);
 #pragma unroll
         for (int i = 0; i < SB; i++) {  // Gentleman-Sande, bits S .. n-1
 #pragma unroll
             for (int j0 = 0; j0 < Q; j0++) {
                 if (j0 & (1 << i)) continue;
                 const int j1 = j0 | (1 << i);
-                const cplx<float> w = tb.iroot[j0 >> (i + 1)];
-                const cplx<float> U = y[j0], V = y[j1];
+                const cplx<T> w = iroot[j0 >> (i + 1)];
+                const cplx<T> U = y[j0], V = y[j1];
                 y[j0] = U + V;
                 y[j1] = (U - V) * w;
             }
         }
-        const long long wgt = 1LL << ba.sched->gshift[l];
+        const long long wgt = ONE ? 1 : 1LL << ba.sched->gshift[l];
 #pragma unroll
         for (int j = 0; j < Q; j++) {
             const int P = (j << S) | c;
-            const cplx<float> z = mk<float>(y[j].x * scale, y[j].y * scale) * untwist[P];
-            const float r0 = rintf(z.x), r1 = rintf(z.y);
-            em = fmaxf(em, fmaxf(fabsf(z.x - r0), fabsf(z.y - r1)));
-            acc[j][0] += (long long)r0 * wgt;
-            acc[j][1] += (long long)r1 * wgt;
+            const cplx<T> z = mk<T>(y[j].x * scale, y[j].y * scale) * untwist[P];
+            const T r0 = rnd(z.x), r1 = rnd(z.y);
+            em = fmaxf(em, rdist(z, r0, r1));
+            if constexpr (ONE) {
+                __builtin_nontemporal_store((long long)r0, o + P);
+                __builtin_nontemporal_store((long long)r1, o + P + L);
+            } else {
+                acc[j][0] += (long long)r0 * wgt;
+                acc[j][1] += (long long)r1 * wgt;
+            }
         }
     }
-    const gptr<long long> o = to_global(ba.out + pr.out_off);
+    if constexpr (ONE) return em;
 #pragma unroll
-    for (int j = 0; j < Q; j++) {
+    for (int j = 0; j < (ONE ? 1 : Q); j++) {
         const int P = (j << S) | c;
         __builtin_nontemporal_store(acc[j][0], o + P);
         __builtin_nontemporal_store(acc[j][1], o + P + L);
@@ -306,66 +370,108 @@ __device__ __forceinline__ float combine_pos(const BArgs& ba, const BProd& pr, c
     return em;
 }
 
-// combine_pos for the two adjacent positions c, c + 1 (c even): one 16-byte
-// load per row, group and untwist pair.
-template <int SB>
+// Elements c and c + 1 (c even) of a cplx<T> array: one 16-byte load for
+// float, two for double.
+template <typename T>
+__device__ __forceinline__ void load_pair_nt(gptr<const cplx<T>> p, cplx<T>& z0, cplx<T>& z1)
+{
+    if constexpr (sizeof(T) == 4) {
+        typedef float f32x4 __attribute__((ext_vector_type(4)));
+        const f32x4 w = __builtin_nontemporal_load((gptr<const f32x4>)p);
+        z0 = mk<float>(w[0], w[1]);
+        z1 = mk<float>(w[2], w[3]);
+    } else {
+        z0 = ldg_nt<T>(p);
+        z1 = ldg_nt<T>(p + 1);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void load_pair(gptr<const cplx<T>> p, cplx<T>& z0, cplx<T>& z1)
+{
+    if constexpr (sizeof(T) == 4) {
+        typedef float f32x4 __attribute__((ext_vector_type(4)));
+        const f32x4 w = *(gptr<const f32x4>)p;
+        z0 = mk<float>(w[0], w[1]);
+        z1 = mk<float>(w[2], w[3]);
+    } else {
+        z0 = p[0];
+        z1 = p[1];
+    }
+}
+
+// combine_pos for the two adjacent positions c, c + 1 (c even): paired
+// loads of every row, group and untwist entry.
+template <typename T, int SB>
 __device__ __forceinline__ float combine_pos2(const BArgs& ba, const BProd& pr, const BTables& tb,
                                               int c)
 {
-    typedef float f32x4 __attribute__((ext_vector_type(4)));
     constexpr int Q = 1 << SB;
+    constexpr bool ONE = sizeof(T) == 8;  // one group: outputs stored as they are rounded
     const int S = pr.S;
     const int L = 1 << pr.n;
-    const float scale = 1.0f / (float)L;
-    const int ngrp = ba.sched->G;
-    long long acc[Q][2][2];  // [row][position][re, im]
+    const T scale = T(1) / (T)L;
+    const int ngrp = ONE ? 1 : ba.sched->G;
+    const gptr<long long> o = to_global(ba.out + pr.out_off);
+    long long acc[ONE ? 1 : Q][2][2];  // [row][position][re, im]
 #pragma unroll
-    for (int j = 0; j < Q; j++) acc[j][0][0] = acc[j][0][1] = acc[j][1][0] = acc[j][1][1] = 0;
+    for (int j = 0; j < (ONE ? 1 : Q); j++) acc[j][0][0] = acc[j][0][1] = acc[j][1][0] = acc[j][1][1] = 0;
     float em = 0.f;
-    const gptr<const cplx<float>> untwist = to_global(tb.untwist);
+    const gptr<const cplx<T>> untwist = to_global(tabp<T>(tb.untwist));
+    const cplx<T>* iroot = tabp<T>(tb.iroot);
     for (int l = 0; l < ngrp; l++) {
-        const gptr<const cplx<float>> src = to_global(ba.scratch + pr.scratch_off + (long long)l * L);
-        cplx<float> y[2][Q];
+        const gptr<const cplx<T>> src =
+            to_global(static_cast<const cplx<T>*>(ba.scratch) + pr.scratch_off + (long long)l * L);
+        cplx<T> y[2][Q];
 #pragma unroll
-        for (int j = 0; j < Q; j++) {
-            const f32x4 w = __builtin_nontemporal_load((gptr<const f32x4>)(src + (j << S) + c));
-            y[0][j] = mk<float>(w[0], w[1]);
-            y[1][j] = mk<float>(w[2], w[3]);
-        }
+        for (int j = 0; j < Q; j++) load_pair_nt<T>(src + (j << S) + c, y[0][j], y[1][j]);
 #pragma unroll
         for (int i = 0; i < SB; i++) {  // Gentleman-Sande, bits S .. n-1
 #pragma unroll
             for (int j0 = 0; j0 < Q; j0++) {
                 if (j0 & (1 << i)) continue;
                 const int j1 = j0 | (1 << i);
-                const cplx<float> w = tb.iroot[j0 >> (i + 1)];
+                const cplx<T> w = iroot[j0 >> (i + 1)];
 #pragma unroll
                 for (int v = 0; v < 2; v++) {
-                    const cplx<float> U = y[v][j0], V = y[v][j1];
+                    const cplx<T> U = y[v][j0], V = y[v][j1];
                     y[v][j0] = U + V;
                     y[v][j1] = (U - V) * w;
                 }
             }
         }
-        const long long wgt = 1LL << ba.sched->gshift[l];
+        const long long wgt = ONE ? 1 : 1LL << ba.sched->gshift[l];
 #pragma unroll
         for (int j = 0; j < Q; j++) {
             const int P = (j << S) | c;
-            const f32x4 ut = *(gptr<const f32x4>)(untwist + P);
+            cplx<T> ut[2];
+            load_pair<T>(untwist + P, ut[0], ut[1]);
+            long long rr[2][2];
 #pragma unroll
             for (int v = 0; v < 2; v++) {
-                const cplx<float> u = v ? mk<float>(ut[2], ut[3]) : mk<float>(ut[0], ut[1]);
-                const cplx<float> z = mk<float>(y[v][j].x * scale, y[v][j].y * scale) * u;
-                const float r0 = rintf(z.x), r1 = rintf(z.y);
-                em = fmaxf(em, fmaxf(fabsf(z.x - r0), fabsf(z.y - r1)));
-                acc[j][v][0] += (long long)r0 * wgt;
-                acc[j][v][1] += (long long)r1 * wgt;
+                const cplx<T> z = mk<T>(y[v][j].x * scale, y[v][j].y * scale) * ut[v];
+                const T r0 = rnd(z.x), r1 = rnd(z.y);
+                em = fmaxf(em, rdist(z, r0, r1));
+                if constexpr (ONE) {
+                    rr[v][0] = (long long)r0;
+                    rr[v][1] = (long long)r1;
+                } else {
+                    acc[j][v][0] += (long long)r0 * wgt;
+                    acc[j][v][1] += (long long)r1 * wgt;
+                }
+            }
+            if constexpr (ONE) {  // (out_offset need not be even: 8-byte stores)
+#pragma unroll
+                for (int v = 0; v < 2; v++) {
+                    __builtin_nontemporal_store(rr[v][0], o + P + v);
+                    __builtin_nontemporal_store(rr[v][1], o + P + v + L);
+                }
             }
         }
     }
-    const gptr<long long> o = to_global(ba.out + pr.out_off);
+    if constexpr (ONE) return em;
 #pragma unroll
-    for (int j = 0; j < Q; j++) {
+    for (int j = 0; j < (ONE ? 1 : Q); j++) {
         const int P = (j << S) | c;
 #pragma unroll
         for (int v = 0; v < 2; v++) {
@@ -376,14 +482,15 @@ __device__ __forceinline__ float combine_pos2(const BArgs& ba, const BProd& pr, 
     return em;
 }
 
+template <typename T>
 __device__ __forceinline__ float combine_pos_rt(const BArgs& ba, const BProd& pr,
                                                 const BTables& tb, int c)
 {
     switch (pr.SB) {
-        case 1: return combine_pos<1>(ba, pr, tb, c);
-        case 2: return combine_pos<2>(ba, pr, tb, c);
-        case 3: return combine_pos<3>(ba, pr, tb, c);
-        case 4: return combine_pos<4>(ba, pr, tb, c);
+        case 1: return combine_pos<T, 1>(ba, pr, tb, c);
+        case 2: return combine_pos<T, 2>(ba, pr, tb, c);
+        case 3: return combine_pos<T, 3>(ba, pr, tb, c);
+        case 4: return combine_pos<T, 4>(ba, pr, tb, c);
         default: return 0.f;
     }
 }
@@ -404,8 +511,8 @@ template <typename V> __device__ __forceinline__ V uniform_copy(const V& x)
 // Stash element e (index in the thread's register layout) of slot s, for
 // thread t: for a fixed e consecutive threads touch consecutive elements.
 // Each thread only reads back what it wrote itself.
-template <int NT>
-__device__ __forceinline__ gptr<cplx<float>> stash_at(gptr<cplx<float>> base, int s, int e, int t)
+template <int NT, typename T>
+__device__ __forceinline__ gptr<cplx<T>> stash_at(gptr<cplx<T>> base, int s, int e, int t)
 {
     return base + ((long long)s << SMAX) + e * NT + t;
 }
@@ -421,8 +528,8 @@ __device__ __forceinline__ gptr<cplx<float>> stash_at(gptr<cplx<float>> base, in
 //      from its slot.
 // An inverse transform ends in emit(): rounding and accumulation into the
 // output (SB = 0) or the store of the sub-result to scratch (SB > 0).
-template <int KIND, int S, int E, int RB, int TILE>
-__device__ __forceinline__ void run_item(cplx<float>* lds, const BArgs& ba, const BProd& pr,
+template <typename T, int KIND, int S, int E, int RB, int TILE>
+__device__ __forceinline__ void run_item(cplx<T>* lds, const BArgs& ba, const BProd& pr,
                                          const BTables& tb, int prod, int q, int tid, int* sh_last)
 {
     constexpr int EPT = 1 << E;
@@ -436,62 +543,97 @@ __device__ __forceinline__ void run_item(cplx<float>* lds, const BArgs& ba, cons
     const int L = 1 << pr.n;
     const BSched& sc = *ba.sched;
 
-    ProdArgs<float> pa{};
+    const cplx<T>* root = tabp<T>(tb.root);
+    const cplx<T>* iroot = tabp<T>(tb.iroot);
+    const cplx<T>* wc = static_cast<const cplx<T>*>(ba.wc);
+    const cplx<T>* wci = static_cast<const cplx<T>*>(ba.wci);
+    ProdArgs<T> pa{};
     pa.n = pr.n;
     pa.S = S;
     pa.nega = 0;
-    pa.tab = tb.root;
-    pa.itab = tb.iroot;
-    pa.twist = tb.twist;
-    PassArgs<float> pf{};
+    pa.tab = root;
+    pa.itab = iroot;
+    pa.twist = tabp<T>(tb.twist);
+    PassArgs<T> pf{};
     pf.n = pr.n;
     pf.K = S;
-    pf.tab = tb.root;
-    PassArgs<float> pinv = pf;
-    pinv.tab = tb.iroot;
+    pf.tab = root;
+    PassArgs<T> pinv = pf;
+    pinv.tab = iroot;
     TileGeo g;
     g.hi_pos = q << S;
     g.blo = 0;
     g.K = S;
     g.logC = 0;
-    g.tab = make_rsrc(tb.root);
+    g.tab = make_rsrc(root);
     TileGeo gi = g;
-    gi.tab = make_rsrc(tb.iroot);
+    gi.tab = make_rsrc(iroot);
 
     // coefficients of row q of the top network, twist factored
-    cplx<float> C[QMAX];
-    if constexpr (QMAX == 1) {
-        top_row_coeffs<float, 0>(pa, q, C);
-    } else {
-        switch (SB) {
-            case 0: top_row_coeffs<float, 0>(pa, q, C); break;
-            case 1: top_row_coeffs<float, 1>(pa, q, C); break;
-            case 2: top_row_coeffs<float, 2>(pa, q, C); break;
-            case 3: top_row_coeffs<float, 3>(pa, q, C); break;
-            default: top_row_coeffs<float, 4>(pa, q, C); break;
+    auto coeffs = [&](cplx<T>* C) {
+        if constexpr (QMAX == 1) {
+            top_row_coeffs<T, 0>(pa, q, C);
+        } else {
+            switch (SB) {
+                case 0: top_row_coeffs<T, 0>(pa, q, C); break;
+                case 1: top_row_coeffs<T, 1>(pa, q, C); break;
+                case 2: top_row_coeffs<T, 2>(pa, q, C); break;
+                case 3: top_row_coeffs<T, 3>(pa, q, C); break;
+                default: top_row_coeffs<T, 4>(pa, q, C); break;
+            }
         }
+    };
+    // FP32: in registers, once per item.  FP64: in LDS beside the tile
+    // (2^SB double coefficients would not fit the registers beside the b
+    // spectrum), with the split twist tables; the phase-1 barrier publishes
+    // them.
+    constexpr bool TWL = sizeof(T) == 8;
+    constexpr bool ONE = sizeof(T) == 8;  // one limb per operand, one group
+    cplx<T> C_item[TWL ? 1 : QMAX];
+    cplx<T>* cl = lds + (1 << TILE);
+    cplx<T>* tws = cl + (1 << SBMAX);
+    if constexpr (TWL) {
+        if (tid < Q) {  // thread j: C[j]
+            cplx<T> cj;
+            if constexpr (QMAX == 1) {
+                cj = top_row_coeff<T, 0>(pa, q, tid);
+            } else {
+                switch (SB) {
+                    case 0: cj = top_row_coeff<T, 0>(pa, q, tid); break;
+                    case 1: cj = top_row_coeff<T, 1>(pa, q, tid); break;
+                    case 2: cj = top_row_coeff<T, 2>(pa, q, tid); break;
+                    case 3: cj = top_row_coeff<T, 3>(pa, q, tid); break;
+                    default: cj = top_row_coeff<T, 4>(pa, q, tid); break;
+                }
+            }
+            cl[tid] = cj;
+        }
+        fill_twist_split<T, S>(tws, pa.twist, tid, BT);
+    } else {
+        coeffs(C_item);
     }
     const gptr<const int> xa = to_global(ba.a + pr.a_off);
     const gptr<const int> xb = to_global(ba.b + pr.b_off);
-    const gptr<const cplx<float>> twist = to_global(tb.twist);
-    const gptr<const cplx<float>> untwist = to_global(tb.untwist);
-    const gptr<cplx<float>> stash =
-        to_global(ba.stash + (((long long)blockIdx.x * ba.stash_slots) << SMAX));
-    const float scale = 1.0f / (float)L;
+    const gptr<const cplx<T>> twist = to_global(tabp<T>(tb.twist));
+    const gptr<const cplx<T>> untwist = to_global(tabp<T>(tb.untwist));
+    const gptr<cplx<T>> stash = to_global(static_cast<cplx<T>*>(ba.stash) +
+                                          (((long long)blockIdx.x * ba.stash_slots) << SMAX));
+    const T scale = T(1) / (T)L;
     float emax = 0.f;
     bool first_out = true;
 
     // gather with a compile-time partner count
     auto gather = [&](gptr<const int> x, int w, int limb, int k, int o, int t) {
+        const cplx<T>* C = TWL ? cl : C_item;
         if constexpr (QMAX == 1) {
-            item_load<KIND, S, 1, E>(lds, twist, x, L, w, limb, k, o, C, t);
+            item_load<T, KIND, S, 1, E, TWL>(lds, twist, x, L, w, limb, k, o, C, t, tws);
         } else {
             switch (SB) {
-                case 0: item_load<KIND, S, 1, E>(lds, twist, x, L, w, limb, k, o, C, t); break;
-                case 1: item_load<KIND, S, 2, E>(lds, twist, x, L, w, limb, k, o, C, t); break;
-                case 2: item_load<KIND, S, 4, E>(lds, twist, x, L, w, limb, k, o, C, t); break;
-                case 3: item_load<KIND, S, 8, E>(lds, twist, x, L, w, limb, k, o, C, t); break;
-                default: item_load<KIND, S, 16, E>(lds, twist, x, L, w, limb, k, o, C, t); break;
+                case 0: item_load<T, KIND, S, 1, E, TWL>(lds, twist, x, L, w, limb, k, o, C, t, tws); break;
+                case 1: item_load<T, KIND, S, 2, E, TWL>(lds, twist, x, L, w, limb, k, o, C, t, tws); break;
+                case 2: item_load<T, KIND, S, 4, E, TWL>(lds, twist, x, L, w, limb, k, o, C, t, tws); break;
+                case 3: item_load<T, KIND, S, 8, E, TWL>(lds, twist, x, L, w, limb, k, o, C, t, tws); break;
+                default: item_load<T, KIND, S, 16, E, TWL>(lds, twist, x, L, w, limb, k, o, C, t, tws); break;
             }
         }
     };
@@ -500,17 +642,17 @@ __device__ __forceinline__ void run_item(cplx<float>* lds, const BArgs& ba, cons
     auto emit = [&](int grp, int u) {
         if (SB == 0) {
             if (active) {
-                const long long wgt = 1LL << __builtin_amdgcn_readfirstlane(sc.gshift[grp]);
+                const long long wgt = ONE ? 1 : 1LL << __builtin_amdgcn_readfirstlane(sc.gshift[grp]);
                 const gptr<long long> o = to_global(ba.out + pr.out_off);
 #pragma unroll 4
                 for (int i = 0; i < EPT; i++) {
                     const int c = u + i * NT;
-                    cplx<float> z = lds[swz(c)];
-                    z = mk<float>(z.x * scale, z.y * scale) * untwist[c];
-                    const float r0 = rintf(z.x), r1 = rintf(z.y);
-                    emax = fmaxf(emax, fmaxf(fabsf(z.x - r0), fabsf(z.y - r1)));
+                    cplx<T> z = lds[swz(c)];
+                    z = mk<T>(z.x * scale, z.y * scale) * untwist[c];
+                    const T r0 = rnd(z.x), r1 = rnd(z.y);
+                    emax = fmaxf(emax, rdist(z, r0, r1));
                     const long long v0 = (long long)r0 * wgt, v1 = (long long)r1 * wgt;
-                    if (first_out) {
+                    if (ONE || first_out) {
                         o[c] = v0;
                         o[c + L] = v1;
                     } else {
@@ -522,37 +664,39 @@ __device__ __forceinline__ void run_item(cplx<float>* lds, const BArgs& ba, cons
             first_out = false;
         } else {
             if (active) {
-                const gptr<cplx<float>> dst = to_global(ba.scratch + pr.scratch_off +
-                                                        (long long)grp * L + ((long long)q << S));
+                const gptr<cplx<T>> dst = to_global(static_cast<cplx<T>*>(ba.scratch) +
+                                                    pr.scratch_off + (long long)grp * L +
+                                                    ((long long)q << S));
 #pragma unroll 4
                 for (int i = 0; i < EPT; i++) {
                     const int c = u + i * NT;
-                    stg_nt(dst + c, lds[swz(c)]);
+                    stg_nt<T>(dst + c, lds[swz(c)]);
                 }
             }
         }
     };
 
     // phase 1: b-limb spectra
-    cplx<float> breg[EPT];
-    for (int jb = 0; jb < ba.kb; jb++) {
+    cplx<T> breg[EPT];
+    const int kb = ONE ? 1 : ba.kb;
+    for (int jb = 0; jb < kb; jb++) {
         int t = tid;
         asm volatile("" : "+v"(t));
         __syncthreads();
         gather(xb, ba.wb, jb, ba.kb, (int)ba.ob, t);
-        run_steps_ct_active<float, MODE_STD_FWD, E, NT, S, RB, 0, NST - 1>(lds, pf, g, t, ba.wc, active);
+        run_steps_ct_active<T, MODE_STD_FWD, E, NT, S, RB, 0, NST - 1>(lds, pf, g, t, wc, active);
         if (active) {
 #pragma unroll
             for (int kk = 0; kk < NG; kk++) {
                 const int rbase = (t + kk * NT) << RB;
-                cplx<float> v[1 << RB];
+                cplx<T> v[1 << RB];
 #pragma unroll
                 for (int j = 0; j < (1 << RB); j++) v[j] = lds[swz(rbase | j)];
-                group_stages_derived<float, MODE_STD_FWD, RB>(v, pf, g, rbase, 0, 0, ba.wc);
+                group_stages_derived<T, MODE_STD_FWD, RB>(v, pf, g, rbase, 0, 0, wc);
 #pragma unroll
                 for (int j = 0; j < (1 << RB); j++) {
                     const int e = (kk << RB) + j;
-                    if (jb == 0)
+                    if (ONE || jb == 0)
                         breg[e] = v[j];
                     else
                         *stash_at<NT>(stash, jb - 1, e, t) = v[j];
@@ -561,13 +705,40 @@ __device__ __forceinline__ void run_item(cplx<float>* lds, const BArgs& ba, cons
         }
     }
 
+    // one limb each (FP64): the a spectrum times the b spectrum in registers,
+    // then the inverse
+    if constexpr (ONE) {
+        int u = tid;
+        asm volatile("" : "+v"(u));
+        __syncthreads();
+        gather(xa, 0, 0, 1, 0, u);
+        run_steps_ct_active<T, MODE_STD_FWD, E, NT, S, RB, 0, NST - 1>(lds, pf, g, u, wc, active);
+        if (active) {
+#pragma unroll
+            for (int kk = 0; kk < NG; kk++) {
+                const int rbase = (u + kk * NT) << RB;
+                cplx<T> v[1 << RB];
+#pragma unroll
+                for (int j = 0; j < (1 << RB); j++) v[j] = lds[swz(rbase | j)];
+                group_stages_derived<T, MODE_STD_FWD, RB>(v, pf, g, rbase, 0, 0, wc);
+#pragma unroll
+                for (int j = 0; j < (1 << RB); j++) v[j] = v[j] * breg[(kk << RB) + j];
+                group_stages_derived<T, MODE_STD_INV, RB>(v, pinv, gi, rbase, 0, 0, wci);
+#pragma unroll
+                for (int j = 0; j < (1 << RB); j++) lds[swz(rbase | j)] = v[j];
+            }
+        }
+        run_steps_ct_active<T, MODE_STD_INV, E, NT, S, RB, 1, NST>(lds, pinv, gi, u, wci, active);
+        emit(0, u);
+    }
+
     // phases 2 and 3
-    for (int ia = 0; ia < ba.ka; ia++) {
+    for (int ia = 0; ia < (ONE ? 0 : ba.ka); ia++) {
         int u = tid;
         asm volatile("" : "+v"(u));
         __syncthreads();
         gather(xa, ba.wa, ia, ba.ka, (int)ba.oa, u);
-        run_steps_ct_active<float, MODE_STD_FWD, E, NT, S, RB, 0, NST - 1>(lds, pf, g, u, ba.wc,
+        run_steps_ct_active<T, MODE_STD_FWD, E, NT, S, RB, 0, NST - 1>(lds, pf, g, u, wc,
                                                                        active);
         const int dg = __builtin_amdgcn_readfirstlane(sc.dgroup[ia]);
         const bool dl = __builtin_amdgcn_readfirstlane(sc.dload[ia]) != 0;
@@ -576,24 +747,24 @@ __device__ __forceinline__ void run_item(cplx<float>* lds, const BArgs& ba, cons
 #pragma unroll
             for (int kk = 0; kk < NG; kk++) {
                 const int rbase = (u + kk * NT) << RB;
-                cplx<float> v[1 << RB];
+                cplx<T> v[1 << RB];
 #pragma unroll
                 for (int j = 0; j < (1 << RB); j++) v[j] = lds[swz(rbase | j)];
-                group_stages_derived<float, MODE_STD_FWD, RB>(v, pf, g, rbase, 0, 0, ba.wc);
+                group_stages_derived<T, MODE_STD_FWD, RB>(v, pf, g, rbase, 0, 0, wc);
                 // products with the b-limb spectra, CH elements at a time;
                 // each round issues all its stash loads before its stores
                 constexpr int CH = (1 << RB) < 8 ? (1 << RB) : 8;
 #pragma unroll
                 for (int j0 = 0; j0 < (1 << RB); j0 += CH) {
                     const int e0 = (kk << RB) + j0;
-                    cplx<float> w[CH];
+                    cplx<T> w[CH];
                     if (dl) {
-                        const gptr<cplx<float>> ds = stash_at<NT>(stash, dslot, e0, u);
+                        const gptr<cplx<T>> ds = stash_at<NT>(stash, dslot, e0, u);
 #pragma unroll
                         for (int jj = 0; jj < CH; jj++) w[jj] = ds[jj * NT];
                     } else {
 #pragma unroll
-                        for (int jj = 0; jj < CH; jj++) w[jj] = mk<float>(0.f, 0.f);
+                        for (int jj = 0; jj < CH; jj++) w[jj] = mk<T>(T(0), T(0));
                     }
                     for (int jb = 0; jb < ba.kb; jb++) {
                         const int k = ia * ba.kb + jb;
@@ -601,13 +772,13 @@ __device__ __forceinline__ void run_item(cplx<float>* lds, const BArgs& ba, cons
                         const bool regs = cg == dg;
                         const bool first = __builtin_amdgcn_readfirstlane(sc.cfirst[k]) != 0;
                         const int cslot = regs ? 0 : __builtin_amdgcn_readfirstlane(sc.gslot[cg]);
-                        const gptr<cplx<float>> slot = stash_at<NT>(stash, cslot, e0, u);
-                        cplx<float> bj[CH], c[CH];
+                        const gptr<cplx<T>> slot = stash_at<NT>(stash, cslot, e0, u);
+                        cplx<T> bj[CH], c[CH];
                         if (jb == 0) {
 #pragma unroll
                             for (int jj = 0; jj < CH; jj++) bj[jj] = breg[e0 + jj];
                         } else {
-                            const gptr<cplx<float>> bsp = stash_at<NT>(stash, jb - 1, e0, u);
+                            const gptr<cplx<T>> bsp = stash_at<NT>(stash, jb - 1, e0, u);
 #pragma unroll
                             for (int jj = 0; jj < CH; jj++) bj[jj] = bsp[jj * NT];
                         }
@@ -620,7 +791,7 @@ __device__ __forceinline__ void run_item(cplx<float>* lds, const BArgs& ba, cons
 #pragma unroll
                             for (int jj = 0; jj < CH; jj++) slot[jj * NT] = c[jj];
                         } else {
-                            cplx<float> old[CH];
+                            cplx<T> old[CH];
 #pragma unroll
                             for (int jj = 0; jj < CH; jj++) old[jj] = slot[jj * NT];
 #pragma unroll
@@ -631,14 +802,14 @@ __device__ __forceinline__ void run_item(cplx<float>* lds, const BArgs& ba, cons
                     for (int jj = 0; jj < CH; jj++) v[j0 + jj] = w[jj];
                 }
                 if (dg >= 0) {
-                    group_stages_derived<float, MODE_STD_INV, RB>(v, pinv, gi, rbase, 0, 0, ba.wci);
+                    group_stages_derived<T, MODE_STD_INV, RB>(v, pinv, gi, rbase, 0, 0, wci);
 #pragma unroll
                     for (int j = 0; j < (1 << RB); j++) lds[swz(rbase | j)] = v[j];
                 }
             }
         }
         if (dg >= 0) {
-            run_steps_ct_active<float, MODE_STD_INV, E, NT, S, RB, 1, NST>(lds, pinv, gi, u, ba.wci,
+            run_steps_ct_active<T, MODE_STD_INV, E, NT, S, RB, 1, NST>(lds, pinv, gi, u, wci,
                                                                          active);
             emit(dg, u);
         }
@@ -651,16 +822,16 @@ __device__ __forceinline__ void run_item(cplx<float>* lds, const BArgs& ba, cons
 #pragma unroll
                 for (int kk = 0; kk < NG; kk++) {
                     const int rbase = (u + kk * NT) << RB;
-                    cplx<float> v[1 << RB];
+                    cplx<T> v[1 << RB];
 #pragma unroll
                     for (int j = 0; j < (1 << RB); j++)
                         v[j] = *stash_at<NT>(stash, gs, (kk << RB) + j, u);
-                    group_stages_derived<float, MODE_STD_INV, RB>(v, pinv, gi, rbase, 0, 0, ba.wci);
+                    group_stages_derived<T, MODE_STD_INV, RB>(v, pinv, gi, rbase, 0, 0, wci);
 #pragma unroll
                     for (int j = 0; j < (1 << RB); j++) lds[swz(rbase | j)] = v[j];
                 }
             }
-            run_steps_ct_active<float, MODE_STD_INV, E, NT, S, RB, 1, NST>(lds, pinv, gi, u, ba.wci,
+            run_steps_ct_active<T, MODE_STD_INV, E, NT, S, RB, 1, NST>(lds, pinv, gi, u, wci,
                                                                          active);
             emit(grp, u);
         }
@@ -682,7 +853,7 @@ __device__ __forceinline__ void run_item(cplx<float>* lds, const BArgs& ba, cons
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             float em = 0.f;
             // 2^S / BT iterations for every thread
-            for (int c = tid; c < (1 << S); c += BT) em = fmaxf(em, combine_pos_rt(ba, pr, tb, c));
+            for (int c = tid; c < (1 << S); c += BT) em = fmaxf(em, combine_pos_rt<T>(ba, pr, tb, c));
             err_update(ba.err, prod, em);
         }
     }
@@ -693,58 +864,63 @@ __device__ __forceinline__ void run_item(cplx<float>* lds, const BArgs& ba, cons
 // over the workgroup, but a call passes them in vector registers, where every
 // buffer load through a table pointer would become a loop over lanes: the
 // callee works on scalar copies of them.
-template <int KIND, int S, int E, int RB, int TILE>
-__device__ __noinline__ void run_item_call(cplx<float>* lds, const BArgs& ba_in,
+template <typename T, int KIND, int S, int E, int RB, int TILE>
+__device__ __noinline__ void run_item_call(cplx<T>* lds, const BArgs& ba_in,
                                            const BProd& pr_in, const BTables& tb_in, int prod,
                                            int q, int tid, int* sh)
 {
     const BArgs ba = uniform_copy(ba_in);
     const BProd pr = uniform_copy(pr_in);
     const BTables tb = uniform_copy(tb_in);
-    run_item<KIND, S, E, RB, TILE>(lds, ba, pr, tb, __builtin_amdgcn_readfirstlane(prod),
-                                 __builtin_amdgcn_readfirstlane(q), tid, sh);
+    run_item<T, KIND, S, E, RB, TILE>(lds, ba, pr, tb, __builtin_amdgcn_readfirstlane(prod),
+                                      __builtin_amdgcn_readfirstlane(q), tid, sh);
 }
 
-template <int KIND, int S, int RB, int TILE>
-__device__ __forceinline__ void dispatch_sb(cplx<float>* lds, const BArgs& ba, const BProd& pr,
+template <typename T, int KIND, int S, int RB, int TILE>
+__device__ __forceinline__ void dispatch_sb(cplx<T>* lds, const BArgs& ba, const BProd& pr,
                                             const BTables& tb, int prod, int q, int tid, int* sh)
 {
     if constexpr (S <= TILE) {
         constexpr int E = S >= 14 ? 5 : (S == 13 ? 4 : 3);  // NT <= 512
         constexpr int R = RB < E ? RB : E;
         if constexpr (S >= TILE - 1)
-            run_item<KIND, S, E, R, TILE>(lds, ba, pr, tb, prod, q, tid, sh);
+            run_item<T, KIND, S, E, R, TILE>(lds, ba, pr, tb, prod, q, tid, sh);
         else
-            run_item_call<KIND, S, E, R, TILE>(lds, ba, pr, tb, prod, q, tid, sh);
+            run_item_call<T, KIND, S, E, R, TILE>(lds, ba, pr, tb, prod, q, tid, sh);
     }
 }
 
-template <int KIND, int RB, int TILE>
-__device__ __forceinline__ void dispatch_s(cplx<float>* lds, const BArgs& ba, const BProd& pr,
+template <typename T, int KIND, int RB, int TILE>
+__device__ __forceinline__ void dispatch_s(cplx<T>* lds, const BArgs& ba, const BProd& pr,
                                            const BTables& tb, int prod, int q, int tid, int* sh)
 {
     switch (pr.S) {
-        case 5: dispatch_sb<KIND, 5, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
-        case 6: dispatch_sb<KIND, 6, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
-        case 7: dispatch_sb<KIND, 7, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
-        case 8: dispatch_sb<KIND, 8, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
-        case 9: dispatch_sb<KIND, 9, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
-        case 10: dispatch_sb<KIND, 10, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
-        case 11: dispatch_sb<KIND, 11, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
-        case 12: dispatch_sb<KIND, 12, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
-        case 13: dispatch_sb<KIND, 13, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
-        case 14: dispatch_sb<KIND, 14, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 5: dispatch_sb<T, KIND, 5, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 6: dispatch_sb<T, KIND, 6, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 7: dispatch_sb<T, KIND, 7, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 8: dispatch_sb<T, KIND, 8, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 9: dispatch_sb<T, KIND, 9, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 10: dispatch_sb<T, KIND, 10, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 11: dispatch_sb<T, KIND, 11, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 12: dispatch_sb<T, KIND, 12, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 13: dispatch_sb<T, KIND, 13, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
+        case 14: dispatch_sb<T, KIND, 14, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh); break;
         default: break;
     }
 }
 
-// TILE = 14: one 128 KiB workgroup per CU.  TILE = 13: 64 KiB workgroups,
-// compiled for 4 waves per SIMD so that two of them share a CU.
-template <int RB, int TILE>
-__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(TILE == 13 ? 4 : 1)))
+// FP32: TILE = 14, one 128 KiB workgroup per CU; TILE = 13, 64 KiB
+// workgroups, compiled for 4 waves per SIMD so that two of them share a CU.
+// FP64: TILE = 13, one 128 KiB workgroup per CU.
+template <typename T, int RB, int TILE>
+__global__ __launch_bounds__(BT)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && TILE == 13 ? 4 : 1)))
 void batched_kernel(BArgs ba)
 {
-    __shared__ cplx<float> lds[1 << TILE];
+    // FP64: the tile, then the item's top-row coefficients (2^SBMAX) and
+    // split twist tables (at most 2^(TILE-7) + 128 entries)
+    constexpr int EXTRA = sizeof(T) == 8 ? (1 << SBMAX) + (1 << (TILE - 7)) + 128 : 0;
+    __shared__ cplx<T> lds[(1 << TILE) + EXTRA];
     __shared__ int sh_item;
     __shared__ int sh_last;
     const int tid = threadIdx.x;
@@ -767,9 +943,9 @@ void batched_kernel(BArgs ba)
             const BProd pr = uniform_copy(ba.prods[item.x]);
             const BTables tb = uniform_copy(ba.tabs[pr.tabset]);
             if (pr.kind == KIND_NEGA)
-                dispatch_s<KIND_NEGA, RB, TILE>(lds, ba, pr, tb, item.x, item.y, tid, &sh_last);
+                dispatch_s<T, KIND_NEGA, RB, TILE>(lds, ba, pr, tb, item.x, item.y, tid, &sh_last);
             else
-                dispatch_s<KIND_LINEAR, RB, TILE>(lds, ba, pr, tb, item.x, item.y, tid, &sh_last);
+                dispatch_s<T, KIND_LINEAR, RB, TILE>(lds, ba, pr, tb, item.x, item.y, tid, &sh_last);
             __syncthreads();
         }
     }
@@ -777,7 +953,7 @@ void batched_kernel(BArgs ba)
 
 // One thread per pair of adjacent positions c, c + 1 < 2^S (S the tile) of
 // one split product (blockIdx.x: the product's index in the split list).
-__global__ __launch_bounds__(CT) void batched_combine_kernel(BArgs ba)
+template <typename T> __global__ __launch_bounds__(CT) void batched_combine_kernel(BArgs ba)
 {
     const int prod = __builtin_amdgcn_readfirstlane(ba.split[blockIdx.x]);
     const BProd pr = uniform_copy(ba.prods[prod]);
@@ -785,11 +961,11 @@ __global__ __launch_bounds__(CT) void batched_combine_kernel(BArgs ba)
     const int c = (blockIdx.y * CT + threadIdx.x) * 2;
     float em;
     switch (pr.SB) {
-        case 1: em = combine_pos2<1>(ba, pr, tb, c); break;
-        case 2: em = combine_pos2<2>(ba, pr, tb, c); break;
-        case 3: em = combine_pos2<3>(ba, pr, tb, c); break;
+        case 1: em = combine_pos2<T, 1>(ba, pr, tb, c); break;
+        case 2: em = combine_pos2<T, 2>(ba, pr, tb, c); break;
+        case 3: em = combine_pos2<T, 3>(ba, pr, tb, c); break;
         default:  // 2^4 rows: one position at a time (registers)
-            em = fmaxf(combine_pos<4>(ba, pr, tb, c), combine_pos<4>(ba, pr, tb, c + 1));
+            em = fmaxf(combine_pos<T, 4>(ba, pr, tb, c), combine_pos<T, 4>(ba, pr, tb, c + 1));
             break;
     }
     err_update(ba.err, prod, em);
@@ -798,6 +974,7 @@ __global__ __launch_bounds__(CT) void batched_combine_kernel(BArgs ba)
 }  // namespace
 
 struct gpufft_batched_plan_s {
+    int prec;         // GPUFFT_PREC_F32_SPLIT (limbs) or GPUFFT_PREC_F64 (one limb each)
     int count;
     int n_items;
     int n_split;      // products with SB > 0
@@ -805,9 +982,9 @@ struct gpufft_batched_plan_s {
     int rb;
     int grid;
     int cmb;          // 1: combine split products in the persistent kernel
-    int tile;         // log2 of the LDS tile: 14 (128 KiB) or 13 (64 KiB, two per CU)
+    int tile;         // log2 of the LDS tile: FP32 14 (128 KiB) or 13 (64 KiB, two per CU); FP64 13
     int max_n;        // largest log2 L of the list
-    long long scratch_elems;  // allocated scratch (cplx<float>)
+    long long scratch_elems;  // allocated scratch (cplx<T>)
     int qbeg[NQ], qlen[NQ];
     BProd* d_prods;
     int2* d_items;
@@ -818,8 +995,8 @@ struct gpufft_batched_plan_s {
     BSched* d_sched;
     int stash_slots;
     int stash_grid;   // workgroups the stash is sized for
-    cplx<float>* stash;
-    cplx<float>* scratch;
+    void* stash;
+    void* scratch;
     void* wc;
     void* wci;
     std::vector<void*> tables;
@@ -830,6 +1007,11 @@ struct gpufft_batched_plan_s {
 };
 
 namespace {
+
+size_t elem_bytes(const gpufft_batched_plan_s* p)
+{
+    return p->prec == GPUFFT_PREC_F64 ? sizeof(cplx<double>) : sizeof(cplx<float>);
+}
 
 void release(gpufft_batched_plan_s* p)
 {
@@ -970,7 +1152,7 @@ int build_layout(gpufft_batched_plan_s* p, int tile, hipStream_t st)
         if (p->scratch) (void)hipFree(p->scratch);
         p->scratch = nullptr;
         p->scratch_elems = 0;
-        hipError_t e = hipMalloc((void**)&p->scratch, sizeof(cplx<float>) * scratch);
+        hipError_t e = hipMalloc(&p->scratch, elem_bytes(p) * scratch);
         if (e != hipSuccess) return fail_hip(e, who);
         p->scratch_elems = scratch;
     }
@@ -987,9 +1169,8 @@ int alloc_stash(gpufft_batched_plan_s* p, int grid)
     if (p->stash_slots == 0 || grid <= p->stash_grid) return GPUFFT_OK;
     // the new stash first: on failure the plan keeps its old stash (and any
     // captured graph that points to it) unchanged and usable
-    cplx<float>* fresh = nullptr;
-    hipError_t e = hipMalloc((void**)&fresh,
-                             sizeof(cplx<float>) * ((size_t)grid * p->stash_slots << SMAX));
+    void* fresh = nullptr;
+    hipError_t e = hipMalloc(&fresh, elem_bytes(p) * ((size_t)grid * p->stash_slots << SMAX));
     if (e != hipSuccess) {
         (void)hipGetLastError();  // the failure is reported here, not by the next launch
         return fail_hip(e, "gpufft_batched_plan: stash");
@@ -1034,8 +1215,8 @@ int launch(gpufft_batched_plan_s* p, const int* a, const int* b, long long* out,
     ba.sched = p->d_sched;
     ba.stash = p->stash;
     ba.stash_slots = p->stash_slots;
-    ba.wc = (const cplx<float>*)p->wc;
-    ba.wci = (const cplx<float>*)p->wci;
+    ba.wc = p->wc;
+    ba.wci = p->wci;
     ba.wa = p->wa;
     ba.ka = p->ka;
     ba.wb = p->wb;
@@ -1049,37 +1230,40 @@ int launch(gpufft_batched_plan_s* p, const int* a, const int* b, long long* out,
     ba.ob = p->kb > 1 ? offset(p->wb, p->kb) : 0;
     ba.err = err;
     const dim3 grid(p->grid), block(BT);
-    if (p->tile == 13) {
+    if (p->prec == GPUFFT_PREC_F64) {
+        if (p->rb == 3)
+            hipLaunchKernelGGL((batched_kernel<double, 3, 13>), grid, block, 0, st, ba);
+        else
+            hipLaunchKernelGGL((batched_kernel<double, 4, 13>), grid, block, 0, st, ba);
+    } else if (p->tile == 13) {
         switch (p->rb) {
-            case 3: hipLaunchKernelGGL((batched_kernel<3, 13>), grid, block, 0, st, ba); break;
-            case 4: hipLaunchKernelGGL((batched_kernel<4, 13>), grid, block, 0, st, ba); break;
-            default: hipLaunchKernelGGL((batched_kernel<5, 13>), grid, block, 0, st, ba); break;
+            case 3: hipLaunchKernelGGL((batched_kernel<float, 3, 13>), grid, block, 0, st, ba); break;
+            case 4: hipLaunchKernelGGL((batched_kernel<float, 4, 13>), grid, block, 0, st, ba); break;
+            default: hipLaunchKernelGGL((batched_kernel<float, 5, 13>), grid, block, 0, st, ba); break;
         }
     } else {
         switch (p->rb) {
-            case 3: hipLaunchKernelGGL((batched_kernel<3, 14>), grid, block, 0, st, ba); break;
-            case 4: hipLaunchKernelGGL((batched_kernel<4, 14>), grid, block, 0, st, ba); break;
-            default: hipLaunchKernelGGL((batched_kernel<5, 14>), grid, block, 0, st, ba); break;
+            case 3: hipLaunchKernelGGL((batched_kernel<float, 3, 14>), grid, block, 0, st, ba); break;
+            case 4: hipLaunchKernelGGL((batched_kernel<float, 4, 14>), grid, block, 0, st, ba); break;
+            default: hipLaunchKernelGGL((batched_kernel<float, 5, 14>), grid, block, 0, st, ba); break;
         }
     }
     GFA_CHECK_LAUNCH(who);
     if (!p->cmb && p->n_split > 0) {
-        hipLaunchKernelGGL(batched_combine_kernel, dim3(p->n_split, (1 << p->tile) / (2 * CT)),
-                           dim3(CT), 0, st, ba);
+        const dim3 cgrid(p->n_split, (1 << p->tile) / (2 * CT));
+        if (p->prec == GPUFFT_PREC_F64)
+            hipLaunchKernelGGL(batched_combine_kernel<double>, cgrid, dim3(CT), 0, st, ba);
+        else
+            hipLaunchKernelGGL(batched_combine_kernel<float>, cgrid, dim3(CT), 0, st, ba);
         GFA_CHECK_LAUNCH(who);
     }
     return GPUFFT_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int gpufft_batched_plan_create(gpufft_batched_plan* out, const gpufft_product_desc* descs, int count,
-                               int a_limb_bits, int a_limbs, int b_limb_bits, int b_limbs,
-                               void* stream)
+int create_plan(gpufft_batched_plan* out, const gpufft_product_desc* descs, int count,
+                int a_limb_bits, int a_limbs, int b_limb_bits, int b_limbs, int prec,
+                void* stream, const char* who)
 {
-    const char* who = "gpufft_batched_plan_create";
     if (!out || !descs) return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": NULL argument");
     *out = nullptr;
     if (count < 1) return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": count < 1");
@@ -1090,13 +1274,16 @@ int gpufft_batched_plan_create(gpufft_batched_plan* out, const gpufft_product_de
         (b_limbs > 1 && (b_limbs - 1) * b_limb_bits > 30))
         return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": invalid limb split");
     hipStream_t st = (hipStream_t)stream;
+    const bool f64 = prec == GPUFFT_PREC_F64;
+    const int tmax = f64 ? SMAX64 : SMAX;  // default (and FP64's only) tile
     gpufft_batched_plan_s* p = new gpufft_batched_plan_s();
+    p->prec = prec;
     p->count = count;
     p->wa = a_limbs > 1 ? a_limb_bits : 0;
     p->ka = a_limbs;
     p->wb = b_limbs > 1 ? b_limb_bits : 0;
     p->kb = b_limbs;
-    p->rb = 5;
+    p->rb = f64 ? 4 : 5;
     p->cmb = 0;
     int dev = 0, cus = 256;
     (void)hipGetDevice(&dev);
@@ -1121,7 +1308,7 @@ int gpufft_batched_plan_create(gpufft_batched_plan* out, const gpufft_product_de
         pr.kind = d.kind == GPUFFT_PRODUCT_NEGACYCLIC ? KIND_NEGA : KIND_LINEAR;
         // NEGACYCLIC: FFNT of ring N (L = N/2); LINEAR: FFNT of ring 2N (L = N)
         pr.n = pr.kind == KIND_NEGA ? d.ring_power - 1 : d.ring_power;
-        if (pr.n < SMIN || pr.n > SMAX + SBMAX) {
+        if (pr.n < SMIN || pr.n > tmax + SBMAX) {
             release(p);
             return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": ring_power out of range");
         }
@@ -1143,26 +1330,27 @@ int gpufft_batched_plan_create(gpufft_batched_plan* out, const gpufft_product_de
             BTables t{};
             void* bufs[4] = {nullptr, nullptr, nullptr, nullptr};
             for (int k = 0; k < 4; k++) {
-                hipError_t e = hipMalloc(&bufs[k], sizeof(cplx<float>) << pr.n);
+                hipError_t e = hipMalloc(&bufs[k], elem_bytes(p) << pr.n);
                 if (e != hipSuccess) {
                     release(p);
                     return fail_hip(e, who);
                 }
                 p->tables.push_back(bufs[k]);
             }
+            auto gen = f64 ? gpufft_generate_table_f64 : gpufft_generate_table_f32;
             int s = 0;
-            s |= gpufft_generate_table_f32(bufs[0], GPUFFT_TABLE_FFNT_ROOT, pr.n, 0, st);
-            s |= gpufft_generate_table_f32(bufs[1], GPUFFT_TABLE_FFNT_ROOT, pr.n, 1, st);
-            s |= gpufft_generate_table_f32(bufs[2], GPUFFT_TABLE_FFNT_TWIST, pr.n, 0, st);
-            s |= gpufft_generate_table_f32(bufs[3], GPUFFT_TABLE_FFNT_TWIST, pr.n, 1, st);
+            s |= gen(bufs[0], GPUFFT_TABLE_FFNT_ROOT, pr.n, 0, st);
+            s |= gen(bufs[1], GPUFFT_TABLE_FFNT_ROOT, pr.n, 1, st);
+            s |= gen(bufs[2], GPUFFT_TABLE_FFNT_TWIST, pr.n, 0, st);
+            s |= gen(bufs[3], GPUFFT_TABLE_FFNT_TWIST, pr.n, 1, st);
             if (s) {
                 release(p);
                 return s;
             }
-            t.root = (const cplx<float>*)bufs[0];
-            t.iroot = (const cplx<float>*)bufs[1];
-            t.twist = (const cplx<float>*)bufs[2];
-            t.untwist = (const cplx<float>*)bufs[3];
+            t.root = bufs[0];
+            t.iroot = bufs[1];
+            t.twist = bufs[2];
+            t.untwist = bufs[3];
             f = tabset.emplace(pr.n, (int)tabs.size()).first;
             tabs.push_back(t);
         }
@@ -1173,19 +1361,20 @@ int gpufft_batched_plan_create(gpufft_batched_plan* out, const gpufft_product_de
     const std::vector<BSched> schedv(1, p->sched);
     int s;
     if ((s = upload(&p->d_tabs, tabs, st)) || (s = upload(&p->d_sched, schedv, st)) ||
-        (s = build_layout(p, SMAX, st)) || (s = alloc_stash(p, p->grid))) {
+        (s = build_layout(p, tmax, st)) || (s = alloc_stash(p, p->grid))) {
         release(p);
         return s;
     }
     hipError_t e = hipMalloc((void**)&p->d_counters, sizeof(int) * (count + NQ));
-    if (e == hipSuccess) e = hipMalloc(&p->wc, sizeof(cplx<float>) * 32);
-    if (e == hipSuccess) e = hipMalloc(&p->wci, sizeof(cplx<float>) * 32);
+    if (e == hipSuccess) e = hipMalloc(&p->wc, elem_bytes(p) * 32);
+    if (e == hipSuccess) e = hipMalloc(&p->wci, elem_bytes(p) * 32);
     if (e != hipSuccess) {
         release(p);
         return fail_hip(e, who);
     }
-    if ((s = gpufft_generate_table_f32(p->wc, GPUFFT_TABLE_STAGE_ROOTS, 5, 0, st)) ||
-        (s = gpufft_generate_table_f32(p->wci, GPUFFT_TABLE_STAGE_ROOTS, 5, 1, st))) {
+    auto gen = f64 ? gpufft_generate_table_f64 : gpufft_generate_table_f32;
+    if ((s = gen(p->wc, GPUFFT_TABLE_STAGE_ROOTS, 5, 0, st)) ||
+        (s = gen(p->wci, GPUFFT_TABLE_STAGE_ROOTS, 5, 1, st))) {
         release(p);
         return s;
     }
@@ -1198,6 +1387,25 @@ int gpufft_batched_plan_create(gpufft_batched_plan* out, const gpufft_product_de
     return ok();
 }
 
+}  // namespace
+
+extern "C" {
+
+int gpufft_batched_plan_create(gpufft_batched_plan* out, const gpufft_product_desc* descs, int count,
+                               int a_limb_bits, int a_limbs, int b_limb_bits, int b_limbs,
+                               void* stream)
+{
+    return create_plan(out, descs, count, a_limb_bits, a_limbs, b_limb_bits, b_limbs,
+                       GPUFFT_PREC_F32_SPLIT, stream, "gpufft_batched_plan_create");
+}
+
+int gpufft_batched_plan_create_f64(gpufft_batched_plan* out, const gpufft_product_desc* descs,
+                                   int count, void* stream)
+{
+    return create_plan(out, descs, count, 0, 1, 0, 1, GPUFFT_PREC_F64, stream,
+                       "gpufft_batched_plan_create_f64");
+}
+
 int gpufft_batched_plan_destroy(gpufft_batched_plan plan)
 {
     release(plan);
@@ -1208,7 +1416,7 @@ int gpufft_batched_plan_set_tuning(gpufft_batched_plan plan, int radix_bits, int
 {
     const char* who = "gpufft_batched_plan_set_tuning";
     if (!plan) return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": plan is NULL");
-    if (radix_bits < 3 || radix_bits > 5 || grid < 1)
+    if (radix_bits < 3 || radix_bits > (plan->prec == GPUFFT_PREC_F64 ? 4 : 5) || grid < 1)
         return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": invalid tuning");
     if (int s = alloc_stash(plan, grid)) return s;
     plan->rb = radix_bits;
@@ -1238,6 +1446,8 @@ int gpufft_batched_plan_set_tile(gpufft_batched_plan plan, int log2_tile)
 {
     const char* who = "gpufft_batched_plan_set_tile";
     if (!plan) return fail(GPUFFT_ERR_NULL_POINTER, std::string(who) + ": plan is NULL");
+    if (plan->prec == GPUFFT_PREC_F64 && log2_tile != SMAX64)
+        return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": FP64 plans have log2_tile 13");
     if (log2_tile != 13 && log2_tile != 14)
         return fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": log2_tile must be 13 or 14");
     if (plan->max_n - log2_tile > SBMAX)

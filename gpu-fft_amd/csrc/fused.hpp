@@ -191,6 +191,20 @@ __device__ __forceinline__ void top_row_coeffs(const ProdArgs<T>& a, int q, cplx
     }
 }
 
+// Coefficient C[j] of top_row_coeffs alone (one thread per j).
+template <typename T, int SB>
+__device__ __forceinline__ cplx<T> top_row_coeff(const ProdArgs<T>& a, int q, int j)
+{
+    constexpr int Q = 1 << SB;
+    cplx<T> wq[SB > 0 ? SB : 1];
+    top_row_twiddles<T, SB>(a, q, wq);
+    const cplx<T> tj = a.twist[j << a.S];
+    cplx<T> e[Q];
+#pragma unroll
+    for (int k = 0; k < Q; k++) e[k] = k == j ? tj : mk<T>(T(0), T(0));
+    return top_forward_row<T, SB>(e, wq);
+}
+
 // Twist factors twist[c], c < 2^S, as twist[c & ~(2^TWB-1)] * twist[c & (2^TWB-1)]
 // (exp(2 pi i c / 2N) is multiplicative in c; both factors are entries of the
 // FFNT twist table), from two small LDS tables: the gather then moves only

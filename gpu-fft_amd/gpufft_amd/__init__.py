@@ -84,6 +84,7 @@ _SIGS = {
     "gpufft_polymul_f64_noalloc": [_P, _P, _P, _I, _I, _P, _P, _P, _P, _P],
     "gpufft_polymul_plan_set_radix_bits": [_P, _I],
     "gpufft_batched_plan_create": [_P, _P, _I, _I, _I, _I, _I, _P],
+    "gpufft_batched_plan_create_f64": [_P, _P, _I, _P],
     "gpufft_batched_plan_destroy": [_P],
     "gpufft_batched_plan_set_tuning": [_P, _I, _I],
     "gpufft_batched_plan_set_combine": [_P, _I],
@@ -368,15 +369,25 @@ def pack_products(kinds, ring_powers):
 
 
 class BatchedPlan:
-    """gpufft_batched_plan: uneven negacyclic / linear products, one launch."""
+    """gpufft_batched_plan: uneven negacyclic / linear products, one launch.
+    precision PREC_F32_SPLIT (limbs as given) or PREC_F64 (one limb each;
+    the limb arguments are then ignored)."""
 
-    def __init__(self, descs, a_limb_bits=5, a_limbs=2, b_limb_bits=5, b_limbs=2, stream=None):
+    def __init__(self, descs, a_limb_bits=5, a_limbs=2, b_limb_bits=5, b_limbs=2, stream=None,
+                 precision=PREC_F32_SPLIT):
         h = ctypes.c_void_p()
-        check(_lib.gpufft_batched_plan_create(ctypes.byref(h), ctypes.byref(descs), len(descs),
-                                              a_limb_bits, a_limbs, b_limb_bits, b_limbs,
-                                              _stream(stream)))
+        if precision == PREC_F64:
+            check(_lib.gpufft_batched_plan_create_f64(ctypes.byref(h), ctypes.byref(descs),
+                                                      len(descs), _stream(stream)))
+        elif precision == PREC_F32_SPLIT:
+            check(_lib.gpufft_batched_plan_create(ctypes.byref(h), ctypes.byref(descs), len(descs),
+                                                  a_limb_bits, a_limbs, b_limb_bits, b_limbs,
+                                                  _stream(stream)))
+        else:
+            raise ValueError("precision must be PREC_F32_SPLIT or PREC_F64")
         self.handle = h
         self.count = len(descs)
+        self.precision = precision
 
     def set_tuning(self, radix_bits, grid):
         check(_lib.gpufft_batched_plan_set_tuning(self.handle, radix_bits, grid))

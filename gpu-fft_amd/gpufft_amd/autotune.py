@@ -121,11 +121,16 @@ def tune_batched(plan, run, stream, key, grids=None, reps=3, use_db=True, db_fil
     combined (0: separate kernel, 1: inside the persistent kernel) and the
     LDS tile (log2 points per workgroup: 14, one workgroup of 8 waves per
     CU, or 13, two per CU).  Tiles the plan refuses (a product would need
-    more than 2^4 sub-problems) are skipped."""
+    more than 2^4 sub-problems) are skipped.  FP64 plans have the one tile
+    13 (one workgroup per CU), radix 2^3 or 2^4, and grids of one or two
+    workgroups per CU."""
     import torch
+
+    from . import PREC_F64
     cus = torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
+    f64 = getattr(plan, "precision", None) == PREC_F64
     ok_tiles = []
-    for t in tiles:
+    for t in tiles if not f64 else (13,):
         try:
             plan.tile = t
             ok_tiles.append(t)
@@ -133,9 +138,14 @@ def tune_batched(plan, run, stream, key, grids=None, reps=3, use_db=True, db_fil
             pass
     cands = []
     for t in ok_tiles:
-        per_cu = 1 if t == 14 else 2
-        gs = grids or (cus * per_cu // 2, cus * per_cu)
-        cands += [(rb, g, cmb, t) for cmb in (0, 1) for rb in (3, 4, 5) for g in gs]
+        if f64:
+            gs = grids or (cus, 2 * cus)
+            radices = (3, 4)
+        else:
+            per_cu = 1 if t == 14 else 2
+            gs = grids or (cus * per_cu // 2, cus * per_cu)
+            radices = (3, 4, 5)
+        cands += [(rb, g, cmb, t) for cmb in (0, 1) for rb in radices for g in gs]
 
     def apply(c):
         if len(c) > 3 and plan.tile != c[3]:

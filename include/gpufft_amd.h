@@ -317,9 +317,17 @@ typedef struct gpufft_batched_plan_s* gpufft_batched_plan;
 int gpufft_batched_plan_create(gpufft_batched_plan* out, const gpufft_product_desc* descs, int count,
                                int a_limb_bits, int a_limbs, int b_limb_bits, int b_limbs,
                                void* stream);
+/* The same list in FP64, one limb per operand: exact rounded products while
+ * the coefficients and the transform's rounding stay well inside FP64
+ * (err_stats shows the margin; rings to 2^17 with 9-bit operands leave
+ * errors near 1e-6).  NEGACYCLIC ring_power 6..18, LINEAR 5..17; LDS tile
+ * 2^13 (128 KiB); radix_bits 3 or 4.  Everything else as above.  (No
+ * counterpart in the reference.) */
+int gpufft_batched_plan_create_f64(gpufft_batched_plan* out, const gpufft_product_desc* descs,
+                                   int count, void* stream);
 int gpufft_batched_plan_destroy(gpufft_batched_plan plan);
-/* radix of the LDS steps (2^3..2^5) and persistent grid size (workgroups);
- * chosen by the autotuner (gpufft_amd.autotune) */
+/* radix of the LDS steps (2^3..2^5; FP64 plans 2^3..2^4) and persistent
+ * grid size (workgroups); chosen by the autotuner (gpufft_amd.autotune) */
 int gpufft_batched_plan_set_tuning(gpufft_batched_plan plan, int radix_bits, int grid);
 /* where split products are combined: 0 = separate streaming kernel (default),
  * 1 = in the persistent kernel (one launch); chosen by the autotuner */

@@ -109,3 +109,13 @@ def test_batched_plan_validation_on_host():
     descs[0].b_offset = 2
     with pytest.raises(ValueError, match="multiples of 4"):
         G.BatchedPlan(descs, stream=0)
+    # FP64 plans: NEGACYCLIC 6..18, LINEAR 5..17 (2^13-point tile, at most
+    # 2^4 sub-problems)
+    for kind, rp in ((G.PRODUCT_NEGACYCLIC, 5), (G.PRODUCT_NEGACYCLIC, 19),
+                     (G.PRODUCT_LINEAR, 4), (G.PRODUCT_LINEAR, 18)):
+        descs, _, _ = G.pack_products([kind], [rp])
+        with pytest.raises(ValueError, match="ring_power"):
+            G.BatchedPlan(descs, stream=0, precision=G.PREC_F64)
+    with pytest.raises(ValueError, match="precision"):
+        G.BatchedPlan(descs, stream=0, precision=7)
+    assert G.lib().gpufft_batched_plan_create_f64(None, None, 1, None) == G.ERR_NULL_POINTER

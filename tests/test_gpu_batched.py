@@ -1,8 +1,8 @@
 """GPU parity of the persistent batched product of uneven ring sizes
 (gpufft_batched_*), bit-exact against the oracle's exact NTT products, for
-both product kinds, with the plain launch, the HIP-graph replay (several
-replays with eager work and synchronisation between them) and every tuning
-candidate."""
+both product kinds and both precisions (FP32 limbs, FP64), with the plain
+launch, the HIP-graph replay (several replays with eager work and
+synchronisation between them) and every tuning candidate."""
 import numpy as np
 import pytest
 
@@ -55,6 +55,78 @@ def test_batched_exact_mixed():
     assert np.array_equal(got, exact)
     info = plan.info()
     assert info["items"] > len(descs)  # some products were split
+
+
+@pytest.mark.parametrize("in_kernel", [False, True])
+@pytest.mark.parametrize("rb,grid", [(3, 256), (4, 256), (4, 512), (3, 16)])
+def test_batched_f64_exact_mixed(rb, grid, in_kernel):
+    """The FP64 plan (one limb per operand, 2^13-point tile, split products
+    up to 2^4 sub-problems) on the mixed list: bit-exact against the NTT for
+    every radix, grid and combine placement."""
+    import torch
+
+    import gpufft_amd as G
+    descs, a, b, exact = _workload(MIX_KINDS, MIX_RP, 0x6770756666740005)
+    plan = G.BatchedPlan(descs, stream=None, precision=G.PREC_F64)
+    assert plan.tile == 13 and plan.info()["radix_bits"] == 4
+    plan.set_tuning(rb, grid)
+    plan.combine_in_kernel = in_kernel
+    da, db = to_dev(a.astype(np.int32)), to_dev(b.astype(np.int32))
+    out = torch.full((len(exact),), -3, dtype=torch.int64, device="cuda")
+    err = torch.zeros(len(descs), dtype=torch.float64, device="cuda")
+    plan.execute(da, db, out, err)
+    e = to_host(err)
+    assert np.all(e < 1e-3), e  # FP64: far from the rounding boundary
+    assert np.array_equal(to_host(out), exact)
+
+
+def test_batched_f64_range_edges_and_limits():
+    """FP64 plan: smallest and largest ring powers of both kinds (NEGACYCLIC
+    6 and 18, LINEAR 5 and 17), full-range int32 coefficients of a small
+    ring (exact in FP64), graph replays; radix 2^5 and the 2^14 tile are
+    refused."""
+    import torch
+
+    import gpufft_amd as G
+    kinds, rps = [0, 1, 0, 1], [6, 5, 18, 17]
+    descs, a, b, exact = _workload(kinds, rps, 0x67707566667400c5)
+    plan = G.BatchedPlan(descs, precision=G.PREC_F64)
+    with pytest.raises(ValueError):
+        plan.set_tuning(5, 256)
+    with pytest.raises(ValueError):
+        plan.tile = 14
+    da, db = to_dev(a.astype(np.int32)), to_dev(b.astype(np.int32))
+    s = torch.cuda.Stream()
+    out = torch.zeros(len(exact), dtype=torch.int64, device="cuda")
+    for rep in range(3):
+        with torch.cuda.stream(s):
+            out.fill_(-1)
+            plan.graph_launch(da, db, out, stream=s)
+        s.synchronize()
+        assert np.array_equal(to_host(out), exact), rep
+        _ = (da.float() * 2).sum().item()
+    # a ring of 64 coefficients with a up to 2^26 and b up to 2^14: products
+    # up to 2^46, exact in FP64 (an FP32 plan would need limbs)
+    descs, n_in, n_out = G.pack_products([G.PRODUCT_NEGACYCLIC, G.PRODUCT_LINEAR], [6, 6])
+    rng = np.random.default_rng(0x67707566667400c6)
+    x = rng.integers(-(1 << 26), 1 << 26, n_in)
+    y = rng.integers(-(1 << 14), 1 << 14, n_in)
+    want = np.zeros(n_out, np.int64)
+    for d in descs:
+        N = 1 << d.ring_power
+        p = np.convolve(x[d.a_offset:d.a_offset + N], y[d.b_offset:d.b_offset + N])
+        if d.kind == G.PRODUCT_NEGACYCLIC:
+            w = p[:N].copy()
+            w[:N - 1] -= p[N:]
+            want[d.out_offset:d.out_offset + N] = w
+        else:
+            want[d.out_offset:d.out_offset + 2 * N - 1] = p
+    plan = G.BatchedPlan(descs, precision=G.PREC_F64)
+    out = torch.full((n_out,), 9, dtype=torch.int64, device="cuda")
+    err = torch.zeros(2, dtype=torch.float64, device="cuda")
+    plan.execute(to_dev(x.astype(np.int32)), to_dev(y.astype(np.int32)), out, err)
+    assert np.all(to_host(err) < 0.1), to_host(err)
+    assert np.array_equal(to_host(out), want)
 
 
 def test_batched_graph_replays():
@@ -247,6 +319,30 @@ def test_batched_autotune_db(tmp_path):
     assert np.array_equal(to_host(out), exact)
 
 
+def test_batched_f64_autotune():
+    """tune_batched on an FP64 plan times radix 3/4 x two grids x combine
+    placement at the one tile 13; the tuned plan stays exact."""
+    import torch
+
+    import gpufft_amd as G
+    from gpufft_amd import autotune as A
+    kinds, rps = [1, 0, 1, 0], [17, 15, 11, 14]
+    descs, a, b, exact = _workload(kinds, rps, 0x67707566667402c5)
+    da, db = to_dev(a.astype(np.int32)), to_dev(b.astype(np.int32))
+    out = torch.zeros(len(exact), dtype=torch.int64, device="cuda")
+    s = torch.cuda.Stream()
+    plan = G.BatchedPlan(descs, precision=G.PREC_F64)
+    with torch.cuda.stream(s):
+        cand, timings = A.tune_batched(plan, lambda: plan.execute(da, db, out, stream=s), s,
+                                       key="test_batched_f64", reps=1, use_db=False)
+        out.zero_()
+        plan.execute(da, db, out, stream=s)
+    s.synchronize()
+    assert len(timings) == 8 and {c[3] for c in timings} == {13}
+    assert {c[0] for c in timings} == {3, 4}
+    assert np.array_equal(to_host(out), exact)
+
+
 def test_batched_failed_retune_keeps_plan_usable():
     """A grid whose stash cannot be allocated is refused; the plan keeps its
     old stash, tuning and captured graph, and still gives exact products."""
@@ -317,22 +413,24 @@ def test_batched_configured_size_4096():
     b = S.uniform(S.SEED_BASE + 4 + 0x200, -256, 256, n_in).astype(np.int32)
     print("batched 4096: exact products on the host", flush=True)
     exact = _exact_products(descs, a, b)
-    plan = G.BatchedPlan(descs, **LIMBS)
-    info = plan.info()
-    assert info["items"] > count  # the large products are split
     da, db = to_dev(a), to_dev(b)
     out = torch.zeros(n_out, dtype=torch.int64, device="cuda")
     err = torch.zeros(count, dtype=torch.float64, device="cuda")
     s = torch.cuda.Stream()
-    for rep in range(2):
-        print("batched 4096: graph replay", rep, flush=True)
-        with torch.cuda.stream(s):
-            out.fill_(-1)
-            plan.graph_launch(da, db, out, err, stream=s)
-        s.synchronize()
-        got = to_host(out)
-        e_host = to_host(err)
-        assert np.all(e_host < 0.5), float(e_host.max())
-        bad = [i for i, d in enumerate(descs)
-               if not np.array_equal(got[d.out_offset:d.out_offset + len(exact[i])], exact[i])]
-        assert not bad, (rep, len(bad), bad[:8])
+    for name, plan in (("f32 split", G.BatchedPlan(descs, **LIMBS)),
+                       ("f64", G.BatchedPlan(descs, precision=G.PREC_F64))):
+        info = plan.info()
+        assert info["items"] > count  # the large products are split
+        for rep in range(2):
+            print("batched 4096:", name, "graph replay", rep, flush=True)
+            with torch.cuda.stream(s):
+                out.fill_(-1)
+                plan.graph_launch(da, db, out, err, stream=s)
+            s.synchronize()
+            got = to_host(out)
+            e_host = to_host(err)
+            assert np.all(e_host < 0.5), (name, float(e_host.max()))
+            bad = [i for i, d in enumerate(descs)
+                   if not np.array_equal(got[d.out_offset:d.out_offset + len(exact[i])], exact[i])]
+            assert not bad, (name, rep, len(bad), bad[:8])
+        plan.close()
