@@ -130,6 +130,7 @@ struct BArgs {
     int wa, ka, wb, kb;
     long long oa, ob;
     double* err;        // optional [products]
+    int out_pair;       // 1: every product's output starts 16-byte aligned (paired int64 stores)
 };
 
 __global__ void zero_ints(int* p, int n)
@@ -309,9 +310,13 @@ template <typename T> __device__ __forceinline__ float rdist(cplx<T> z, T r0, T 
 // recombination; writes the unfolded outputs out[P], out[P + L] for
 // P = j 2^S + c.  Returns the largest distance of a group output to its
 // rounding.
-template <typename T, int SB>
+// UL: the untwist factor of P = j 2^S + c is uhi[4 j + ((c - cbase) >> 7)]
+// ulo[c & 127] from LDS (the combine workgroup's 512 positions from cbase),
+// else untwist[P] from the table.
+template <typename T, int SB, bool UL = false>
 __device__ __forceinline__ float combine_pos(const BArgs& ba, const BProd& pr, const BTables& tb,
-                                             int c)
+                                             int c, const cplx<T>* uhi = nullptr,
+                                             const cplx<T>* ulo = nullptr, int cbase = 0)
 {
     constexpr int Q = 1 << SB;
     constexpr bool ONE = sizeof(T) == 8;  // one group: outputs stored as they are rounded
@@ -348,7 +353,12 @@ __device__ __forceinline__ float combine_pos(const BArgs& ba, const BProd& pr, c
 #pragma unroll
         for (int j = 0; j < Q; j++) {
             const int P = (j << S) | c;
-            const cplx<T> z = mk<T>(y[j].x * scale, y[j].y * scale) * untwist[P];
+            cplx<T> ut;
+            if constexpr (UL)
+                ut = uhi[4 * j + ((c - cbase) >> 7)] * ulo[c & 127];
+            else
+                ut = untwist[P];
+            const cplx<T> z = mk<T>(y[j].x * scale, y[j].y * scale) * ut;
             const T r0 = rnd(z.x), r1 = rnd(z.y);
             em = fmaxf(em, rdist(z, r0, r1));
             if constexpr (ONE) {
@@ -400,11 +410,37 @@ __device__ __forceinline__ void load_pair(gptr<const cplx<T>> p, cplx<T>& z0, cp
     }
 }
 
+// Outputs of positions P, P + 1 (P even): r[v][0] to o[P + v] and r[v][1] to
+// o[P + v + L]; one 16-byte store per pair when the output is 16-byte
+// aligned (8-byte stores of every other element leave half-written lines to
+// the memory system), else two 8-byte stores.
+__device__ __forceinline__ void store_pair(gptr<long long> o, int P, int L, const long long (*r)[2],
+                                           int paired)
+{
+    if (paired) {
+        typedef long long i64x2 __attribute__((ext_vector_type(2)));
+        i64x2 re, im;
+        re.x = r[0][0];
+        re.y = r[1][0];
+        im.x = r[0][1];
+        im.y = r[1][1];
+        __builtin_nontemporal_store(re, (gptr<i64x2>)(o + P));
+        __builtin_nontemporal_store(im, (gptr<i64x2>)(o + P + L));
+    } else {
+#pragma unroll
+        for (int v = 0; v < 2; v++) {
+            __builtin_nontemporal_store(r[v][0], o + P + v);
+            __builtin_nontemporal_store(r[v][1], o + P + v + L);
+        }
+    }
+}
+
 // combine_pos for the two adjacent positions c, c + 1 (c even): paired
 // loads of every row, group and untwist entry.
-template <typename T, int SB>
+template <typename T, int SB, bool UL = false>
 __device__ __forceinline__ float combine_pos2(const BArgs& ba, const BProd& pr, const BTables& tb,
-                                              int c)
+                                              int c, const cplx<T>* uhi = nullptr,
+                                              const cplx<T>* ulo = nullptr, int cbase = 0)
 {
     constexpr int Q = 1 << SB;
     constexpr bool ONE = sizeof(T) == 8;  // one group: outputs stored as they are rounded
@@ -445,7 +481,13 @@ __device__ __forceinline__ float combine_pos2(const BArgs& ba, const BProd& pr, 
         for (int j = 0; j < Q; j++) {
             const int P = (j << S) | c;
             cplx<T> ut[2];
-            load_pair<T>(untwist + P, ut[0], ut[1]);
+            if constexpr (UL) {
+                const cplx<T> h = uhi[4 * j + ((c - cbase) >> 7)];
+                ut[0] = h * ulo[c & 127];
+                ut[1] = h * ulo[(c & 127) + 1];
+            } else {
+                load_pair<T>(untwist + P, ut[0], ut[1]);
+            }
             long long rr[2][2];
 #pragma unroll
             for (int v = 0; v < 2; v++) {
@@ -460,25 +502,12 @@ __device__ __forceinline__ float combine_pos2(const BArgs& ba, const BProd& pr, 
                     acc[j][v][1] += (long long)r1 * wgt;
                 }
             }
-            if constexpr (ONE) {  // (out_offset need not be even: 8-byte stores)
-#pragma unroll
-                for (int v = 0; v < 2; v++) {
-                    __builtin_nontemporal_store(rr[v][0], o + P + v);
-                    __builtin_nontemporal_store(rr[v][1], o + P + v + L);
-                }
-            }
+            if constexpr (ONE) store_pair(o, P, L, rr, ba.out_pair);
         }
     }
     if constexpr (ONE) return em;
 #pragma unroll
-    for (int j = 0; j < (ONE ? 1 : Q); j++) {
-        const int P = (j << S) | c;
-#pragma unroll
-        for (int v = 0; v < 2; v++) {
-            __builtin_nontemporal_store(acc[j][v][0], o + P + v);
-            __builtin_nontemporal_store(acc[j][v][1], o + P + v + L);
-        }
-    }
+    for (int j = 0; j < (ONE ? 1 : Q); j++) store_pair(o, (j << S) | c, L, acc[j], ba.out_pair);
     return em;
 }
 
@@ -953,19 +982,35 @@ void batched_kernel(BArgs ba)
 
 // One thread per pair of adjacent positions c, c + 1 < 2^S (S the tile) of
 // one split product (blockIdx.x: the product's index in the split list).
+// FP64: the untwist factors come from LDS, untwist[P] = untwist[P & ~127]
+// untwist[P & 127] (exp(-2 pi i P / 2N) is multiplicative in P; the FP32
+// form keeps the direct table entries, whose rounding its limb error
+// statistics were measured with).
 template <typename T> __global__ __launch_bounds__(CT) void batched_combine_kernel(BArgs ba)
 {
+    constexpr bool UL = sizeof(T) == 8;
+    __shared__ cplx<T> ulo[UL ? 128 : 1];
+    __shared__ cplx<T> uhi[UL ? 4 << SBMAX : 1];
     const int prod = __builtin_amdgcn_readfirstlane(ba.split[blockIdx.x]);
     const BProd pr = uniform_copy(ba.prods[prod]);
     const BTables tb = uniform_copy(ba.tabs[pr.tabset]);
-    const int c = (blockIdx.y * CT + threadIdx.x) * 2;
+    const int cbase = blockIdx.y * CT * 2;  // this workgroup's 512 positions (4 blocks of 128)
+    const int c = cbase + threadIdx.x * 2;
+    if constexpr (UL) {
+        const cplx<T>* ut = tabp<T>(tb.untwist);
+        const int t = threadIdx.x;
+        if (t < 128) ulo[t] = ut[t];
+        if (t < (4 << pr.SB)) uhi[t] = ut[((t >> 2) << pr.S) | (cbase + ((t & 3) << 7))];
+        __syncthreads();
+    }
     float em;
     switch (pr.SB) {
-        case 1: em = combine_pos2<T, 1>(ba, pr, tb, c); break;
-        case 2: em = combine_pos2<T, 2>(ba, pr, tb, c); break;
-        case 3: em = combine_pos2<T, 3>(ba, pr, tb, c); break;
-        default:  // 2^4 rows: one position at a time (registers)
-            em = fmaxf(combine_pos<T, 4>(ba, pr, tb, c), combine_pos<T, 4>(ba, pr, tb, c + 1));
+        case 1: em = combine_pos2<T, 1, UL>(ba, pr, tb, c, uhi, ulo, cbase); break;
+        case 2: em = combine_pos2<T, 2, UL>(ba, pr, tb, c, uhi, ulo, cbase); break;
+        case 3: em = combine_pos2<T, 3, UL>(ba, pr, tb, c, uhi, ulo, cbase); break;
+        default:  // 2^4 rows: one position at a time (registers), consecutive per lane
+            em = fmaxf(combine_pos<T, 4, UL>(ba, pr, tb, cbase + threadIdx.x, uhi, ulo, cbase),
+                       combine_pos<T, 4, UL>(ba, pr, tb, cbase + CT + threadIdx.x, uhi, ulo, cbase));
             break;
     }
     err_update(ba.err, prod, em);
@@ -984,6 +1029,7 @@ struct gpufft_batched_plan_s {
     int cmb;          // 1: combine split products in the persistent kernel
     int tile;         // log2 of the LDS tile: FP32 14 (128 KiB) or 13 (64 KiB, two per CU); FP64 13
     int max_n;        // largest log2 L of the list
+    int out_even;     // every out_offset is even
     long long scratch_elems;  // allocated scratch (cplx<T>)
     int qbeg[NQ], qlen[NQ];
     BProd* d_prods;
@@ -1229,6 +1275,7 @@ int launch(gpufft_batched_plan_s* p, const int* a, const int* b, long long* out,
     ba.oa = p->ka > 1 ? offset(p->wa, p->ka) : 0;
     ba.ob = p->kb > 1 ? offset(p->wb, p->kb) : 0;
     ba.err = err;
+    ba.out_pair = p->out_even && ((uintptr_t)out & 15) == 0;
     const dim3 grid(p->grid), block(BT);
     if (p->prec == GPUFFT_PREC_F64) {
         if (p->rb == 3)
@@ -1325,6 +1372,8 @@ int create_plan(gpufft_batched_plan* out, const gpufft_product_desc* descs, int 
         pr.b_off = d.b_offset;
         pr.N = 1 << d.ring_power;
         pr.out_off = d.out_offset;
+        if (i == 0) p->out_even = 1;
+        if (d.out_offset & 1) p->out_even = 0;
         auto f = tabset.find(pr.n);
         if (f == tabset.end()) {
             BTables t{};
