@@ -271,6 +271,75 @@ __device__ __forceinline__ void item_load(cplx<T>* lds, gptr<const cplx<T>> twis
     }
 }
 
+// FP64 full tiles (S = the tile, S mod RB = 1): item_load with the short
+// top forward step (bit S-1) in registers.  A lane's blocks i and i + NV/2
+// of 4 positions are 2^(S-1) apart, the two halves of that step; partners
+// are accumulated 4 at a time to bound the registers of the loads.  C and
+// the twist factors come from LDS as in item_load<TWL>.
+template <typename T, int KIND, int S, int Q, int E>
+__device__ __forceinline__ void item_load_top(cplx<T>* lds, gptr<const int> x, int L,
+                                              const cplx<T>* C, const cplx<T>* tws,
+                                              const ProdArgs<T>& pa, int q, int tid)
+{
+    typedef int i32x4 __attribute__((ext_vector_type(4)));
+    constexpr int NT = (1 << S) >> E;
+    constexpr int NV = (1 << E) / 4;  // blocks of 4 positions per lane
+    constexpr int NH = NV / 2;        // block i pairs with block i + NH
+    constexpr int JC = Q < 4 ? Q : 4;  // partners per load round
+    static_assert(NH >= 1, "two blocks per lane");
+    if (tid >= NT) return;
+#pragma unroll 1
+    for (int i = 0; i < NH; i++) {
+        int c0[2];
+        c0[0] = (tid + i * NT) * 4;
+        c0[1] = (tid + (i + NH) * NT) * 4;
+        T ax[2][4], ay[2][4];
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) ax[h][v] = ay[h][v] = T(0);
+#pragma unroll
+        for (int j0 = 0; j0 < Q; j0 += JC) {
+            i32x4 xr[2][JC], xi[2][JC];
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int jj = 0; jj < JC; jj++) {
+                    const int P = ((j0 + jj) << S) | c0[h];
+                    xr[h][jj] = *(gptr<const i32x4>)(x + P);
+                    if constexpr (KIND == KIND_NEGA) xi[h][jj] = *(gptr<const i32x4>)(x + P + L);
+                }
+#pragma unroll
+            for (int jj = 0; jj < JC; jj++) {
+                const cplx<T> cj = C[j0 + jj];
+#pragma unroll
+                for (int h = 0; h < 2; h++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        const T re = (T)xr[h][jj][v];
+                        if constexpr (KIND == KIND_NEGA) {
+                            const T im = (T)xi[h][jj][v];
+                            ax[h][v] = ax[h][v] + cj.x * re - cj.y * im;
+                            ay[h][v] = ay[h][v] + cj.x * im + cj.y * re;
+                        } else {
+                            ax[h][v] = ax[h][v] + cj.x * re;
+                            ay[h][v] = ay[h][v] + cj.y * re;
+                        }
+                    }
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            cplx<T> z[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) z[h] = twist_of<T, S>(tws, c0[h] + v) * mk<T>(ax[h][v], ay[h][v]);
+            top_ct<T, S, 1>(z, pa, q);
+#pragma unroll
+            for (int h = 0; h < 2; h++) lds[swz(c0[h] + v)] = z[h];
+        }
+    }
+}
+
 // Streaming load and store of one element through a global pointer.
 template <typename T> __device__ __forceinline__ cplx<T> ldg_nt(gptr<const cplx<T>> p)
 {
@@ -618,6 +687,13 @@ __device__ __forceinline__ void run_item(cplx<T>* lds, const BArgs& ba, const BP
     // them.
     constexpr bool TWL = sizeof(T) == 8;
     constexpr bool ONE = sizeof(T) == 8;  // one limb per operand, one group
+    // FP64 full tiles: the short top step (S mod RB = 1 bit) in registers, in
+    // the gather (forward) and in emit (inverse), one LDS round trip fewer
+    // per transform
+    constexpr int HBT = (ONE && S == TILE) ? S % RB : 0;
+    static_assert(HBT <= 1, "one-bit register top step");
+    constexpr int FS0 = HBT ? 1 : 0;           // first forward step through LDS
+    constexpr int IEND = HBT ? NST - 1 : NST;  // end of the inverse steps through LDS
     cplx<T> C_item[TWL ? 1 : QMAX];
     cplx<T>* cl = lds + (1 << TILE);
     cplx<T>* tws = cl + (1 << SBMAX);
@@ -654,6 +730,16 @@ __device__ __forceinline__ void run_item(cplx<T>* lds, const BArgs& ba, const BP
     // gather with a compile-time partner count
     auto gather = [&](gptr<const int> x, int w, int limb, int k, int o, int t) {
         const cplx<T>* C = TWL ? cl : C_item;
+        if constexpr (HBT) {
+            switch (SB) {
+                case 0: item_load_top<T, KIND, S, 1, E>(lds, x, L, C, tws, pa, q, t); break;
+                case 1: item_load_top<T, KIND, S, 2, E>(lds, x, L, C, tws, pa, q, t); break;
+                case 2: item_load_top<T, KIND, S, 4, E>(lds, x, L, C, tws, pa, q, t); break;
+                case 3: item_load_top<T, KIND, S, 8, E>(lds, x, L, C, tws, pa, q, t); break;
+                default: item_load_top<T, KIND, S, 16, E>(lds, x, L, C, tws, pa, q, t); break;
+            }
+            return;
+        }
         if constexpr (QMAX == 1) {
             item_load<T, KIND, S, 1, E, TWL>(lds, twist, x, L, w, limb, k, o, C, t, tws);
         } else {
@@ -669,6 +755,46 @@ __device__ __forceinline__ void run_item(cplx<T>* lds, const BArgs& ba, const BP
 
     // the inverse of group grp is in LDS
     auto emit = [&](int grp, int u) {
+        if constexpr (HBT) {
+            // element pairs c, c + 2^(S-1): the top inverse step, then the
+            // output (SB = 0) or the scratch sub-result
+            constexpr int NI = EPT / 2;
+            if (!active) return;
+            if (SB == 0) {
+                const gptr<long long> o = to_global(ba.out + pr.out_off);
+#pragma unroll 2
+                for (int i = 0; i < NI; i++) {
+                    const int c = u + i * NT;
+                    cplx<T> z[2];
+#pragma unroll
+                    for (int m = 0; m < 2; m++) z[m] = lds[swz(c + (m << (S - 1)))];
+                    top_gs<T, S, 1>(z, pa, q);
+#pragma unroll
+                    for (int m = 0; m < 2; m++) {
+                        const int cm = c + (m << (S - 1));
+                        const cplx<T> y = mk<T>(z[m].x * scale, z[m].y * scale) * untwist[cm];
+                        const T r0 = rnd(y.x), r1 = rnd(y.y);
+                        emax = fmaxf(emax, rdist(y, r0, r1));
+                        o[cm] = (long long)r0;
+                        o[cm + L] = (long long)r1;
+                    }
+                }
+            } else {
+                const gptr<cplx<T>> dst = to_global(static_cast<cplx<T>*>(ba.scratch) +
+                                                    pr.scratch_off + ((long long)q << S));
+#pragma unroll 2
+                for (int i = 0; i < NI; i++) {
+                    const int c = u + i * NT;
+                    cplx<T> z[2];
+#pragma unroll
+                    for (int m = 0; m < 2; m++) z[m] = lds[swz(c + (m << (S - 1)))];
+                    top_gs<T, S, 1>(z, pa, q);
+#pragma unroll
+                    for (int m = 0; m < 2; m++) stg_nt<T>(dst + c + (m << (S - 1)), z[m]);
+                }
+            }
+            return;
+        }
         if (SB == 0) {
             if (active) {
                 const long long wgt = ONE ? 1 : 1LL << __builtin_amdgcn_readfirstlane(sc.gshift[grp]);
@@ -713,7 +839,7 @@ __device__ __forceinline__ void run_item(cplx<T>* lds, const BArgs& ba, const BP
         asm volatile("" : "+v"(t));
         __syncthreads();
         gather(xb, ba.wb, jb, ba.kb, (int)ba.ob, t);
-        run_steps_ct_active<T, MODE_STD_FWD, E, NT, S, RB, 0, NST - 1>(lds, pf, g, t, wc, active);
+        run_steps_ct_active<T, MODE_STD_FWD, E, NT, S, RB, FS0, NST - 1>(lds, pf, g, t, wc, active);
         if (active) {
 #pragma unroll
             for (int kk = 0; kk < NG; kk++) {
@@ -741,7 +867,7 @@ __device__ __forceinline__ void run_item(cplx<T>* lds, const BArgs& ba, const BP
         asm volatile("" : "+v"(u));
         __syncthreads();
         gather(xa, 0, 0, 1, 0, u);
-        run_steps_ct_active<T, MODE_STD_FWD, E, NT, S, RB, 0, NST - 1>(lds, pf, g, u, wc, active);
+        run_steps_ct_active<T, MODE_STD_FWD, E, NT, S, RB, FS0, NST - 1>(lds, pf, g, u, wc, active);
         if (active) {
 #pragma unroll
             for (int kk = 0; kk < NG; kk++) {
@@ -757,7 +883,7 @@ __device__ __forceinline__ void run_item(cplx<T>* lds, const BArgs& ba, const BP
                 for (int j = 0; j < (1 << RB); j++) lds[swz(rbase | j)] = v[j];
             }
         }
-        run_steps_ct_active<T, MODE_STD_INV, E, NT, S, RB, 1, NST>(lds, pinv, gi, u, wci, active);
+        run_steps_ct_active<T, MODE_STD_INV, E, NT, S, RB, 1, IEND>(lds, pinv, gi, u, wci, active);
         emit(0, u);
     }
 
@@ -767,7 +893,7 @@ __device__ __forceinline__ void run_item(cplx<T>* lds, const BArgs& ba, const BP
         asm volatile("" : "+v"(u));
         __syncthreads();
         gather(xa, ba.wa, ia, ba.ka, (int)ba.oa, u);
-        run_steps_ct_active<T, MODE_STD_FWD, E, NT, S, RB, 0, NST - 1>(lds, pf, g, u, wc,
+        run_steps_ct_active<T, MODE_STD_FWD, E, NT, S, RB, FS0, NST - 1>(lds, pf, g, u, wc,
                                                                        active);
         const int dg = __builtin_amdgcn_readfirstlane(sc.dgroup[ia]);
         const bool dl = __builtin_amdgcn_readfirstlane(sc.dload[ia]) != 0;
@@ -838,7 +964,7 @@ __device__ __forceinline__ void run_item(cplx<T>* lds, const BArgs& ba, const BP
             }
         }
         if (dg >= 0) {
-            run_steps_ct_active<T, MODE_STD_INV, E, NT, S, RB, 1, NST>(lds, pinv, gi, u, wci,
+            run_steps_ct_active<T, MODE_STD_INV, E, NT, S, RB, 1, IEND>(lds, pinv, gi, u, wci,
                                                                          active);
             emit(dg, u);
         }
@@ -860,7 +986,7 @@ __device__ __forceinline__ void run_item(cplx<T>* lds, const BArgs& ba, const BP
                     for (int j = 0; j < (1 << RB); j++) lds[swz(rbase | j)] = v[j];
                 }
             }
-            run_steps_ct_active<T, MODE_STD_INV, E, NT, S, RB, 1, NST>(lds, pinv, gi, u, wci,
+            run_steps_ct_active<T, MODE_STD_INV, E, NT, S, RB, 1, IEND>(lds, pinv, gi, u, wci,
                                                                          active);
             emit(grp, u);
         }
