@@ -1038,7 +1038,11 @@ __device__ __forceinline__ void dispatch_sb(cplx<T>* lds, const BArgs& ba, const
     if constexpr (S <= TILE) {
         constexpr int E = S >= 14 ? 5 : (S == 13 ? 4 : 3);  // NT <= 512
         constexpr int R = RB < E ? RB : E;
-        if constexpr (S >= TILE - 1)
+        // FP64: every item inlined.  A call takes the address of the
+        // argument structures, which then live in private memory: each wave
+        // writes its own copy at entry and per item, and reloads fields from
+        // it, through the vector memory path.
+        if constexpr (S >= TILE - 1 || sizeof(T) == 8)
             run_item<T, KIND, S, E, R, TILE>(lds, ba, pr, tb, prod, q, tid, sh);
         else
             run_item_call<T, KIND, S, E, R, TILE>(lds, ba, pr, tb, prod, q, tid, sh);
