@@ -1014,21 +1014,47 @@ __device__ __forceinline__ void run_item(cplx<T>* lds, const BArgs& ba, const BP
     }
 }
 
-// Small items (S <= 12) as calls, which keeps the kernel's code size and
-// compile time in bounds; the large ones inlined.  The arguments are uniform
-// over the workgroup, but a call passes them in vector registers, where every
-// buffer load through a table pointer would become a loop over lanes: the
-// callee works on scalar copies of them.
-template <typename T, int KIND, int S, int E, int RB, int TILE>
-__device__ __noinline__ void run_item_call(cplx<T>* lds, const BArgs& ba_in,
-                                           const BProd& pr_in, const BTables& tb_in, int prod,
-                                           int q, int tid, int* sh)
+// The kernel's argument structure in the kernel-argument segment (it is
+// the kernel's only argument, at offset 0).  Calls take it from there: a
+// reference to the by-value argument would make every wave keep a private
+// copy of it, written and reloaded through the vector memory path.
+typedef __attribute__((address_space(4))) const BArgs* kargs_t;
+
+__device__ __forceinline__ kargs_t kernel_args()
 {
-    const BArgs ba = uniform_copy(ba_in);
-    const BProd pr = uniform_copy(pr_in);
-    const BTables tb = uniform_copy(tb_in);
-    run_item<T, KIND, S, E, RB, TILE>(lds, ba, pr, tb, __builtin_amdgcn_readfirstlane(prod),
-                                      __builtin_amdgcn_readfirstlane(q), tid, sh);
+    return (kargs_t)__builtin_amdgcn_kernarg_segment_ptr();
+}
+
+// A workgroup-uniform structure from constant memory (scalar loads).
+template <typename V> __device__ __forceinline__ V load_uniform(__attribute__((address_space(4))) const V* p)
+{
+    static_assert(sizeof(V) % 4 == 0, "whole dwords");
+    typedef __attribute__((address_space(4))) const unsigned* cu32;
+    const cu32 w = (cu32)p;
+    unsigned v[sizeof(V) / 4];
+#pragma unroll
+    for (unsigned i = 0; i < sizeof(V) / 4; i++) v[i] = w[i];
+    V r;
+    __builtin_memcpy(&r, v, sizeof(V));
+    return r;
+}
+
+// Small items (S <= 12) as calls, which keeps the kernel's code size and
+// compile time in bounds; the large ones inlined.  The callee takes only
+// scalars: the arguments from the kernel-argument segment, the item's
+// product and table records from memory by index, each made a scalar copy
+// (a buffer load through a table pointer held in vector registers would
+// become a loop over lanes).
+template <typename T, int KIND, int S, int E, int RB, int TILE>
+__device__ __noinline__ void run_item_call(cplx<T>* lds, kargs_t kba, int prod_in, int q_in,
+                                           int tid, int* sh)
+{
+    const BArgs ba = load_uniform(kba);
+    const int prod = __builtin_amdgcn_readfirstlane(prod_in);
+    const int q = __builtin_amdgcn_readfirstlane(q_in);
+    const BProd pr = uniform_copy(ba.prods[prod]);
+    const BTables tb = uniform_copy(ba.tabs[pr.tabset]);
+    run_item<T, KIND, S, E, RB, TILE>(lds, ba, pr, tb, prod, q, tid, sh);
 }
 
 template <typename T, int KIND, int S, int RB, int TILE>
@@ -1038,14 +1064,12 @@ __device__ __forceinline__ void dispatch_sb(cplx<T>* lds, const BArgs& ba, const
     if constexpr (S <= TILE) {
         constexpr int E = S >= 14 ? 5 : (S == 13 ? 4 : 3);  // NT <= 512
         constexpr int R = RB < E ? RB : E;
-        // FP64: every item inlined.  A call takes the address of the
-        // argument structures, which then live in private memory: each wave
-        // writes its own copy at entry and per item, and reloads fields from
-        // it, through the vector memory path.
+        // FP64: every item inlined (the FP32 kernel's larger code keeps the
+        // small items as calls)
         if constexpr (S >= TILE - 1 || sizeof(T) == 8)
             run_item<T, KIND, S, E, R, TILE>(lds, ba, pr, tb, prod, q, tid, sh);
         else
-            run_item_call<T, KIND, S, E, R, TILE>(lds, ba, pr, tb, prod, q, tid, sh);
+            run_item_call<T, KIND, S, E, R, TILE>(lds, kernel_args(), prod, q, tid, sh);
     }
 }
 
