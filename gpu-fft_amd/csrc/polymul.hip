@@ -359,7 +359,10 @@ int gpufft_polymul_plan_create(gpufft_polymul_plan* out, int ring_power, int pre
     // FP32: 1024 threads of 16 points per full 2^14-point tile (measured
     // faster than 512 threads of 32 at N = 2^16: 0.38 against 0.45 ms per
     // batch of 256), 512 threads of 32 points below the full tile
-    pl->e = precision == GPUFFT_PREC_F64 ? GFA_E64 : (pl->S == F32_MAX_S ? 4 : 5);
+    // FP32: 16 points per thread (1024 threads) for full tiles of products
+    // in one or two sub-problems; with more, the top-row gather's partner
+    // loads spill at 128 VGPRs and 32 points per thread (256 VGPRs) is faster
+    pl->e = precision == GPUFFT_PREC_F64 ? GFA_E64 : (pl->S == F32_MAX_S && pl->SB <= 1 ? 4 : 5);
     pl->cmb = 0;  // separate pass: measured faster at N = 2^16, batch 256
     hipStream_t st = (hipStream_t)stream;
     int s = precision == GPUFFT_PREC_F64 ? make_tables<double>(pl, st) : make_tables<float>(pl, st);

@@ -196,9 +196,10 @@ size_t gpufft_polymul_plan_scratch_bytes(gpufft_polymul_plan plan);
 int gpufft_polymul_plan_set_radix_bits(gpufft_polymul_plan plan, int radix_bits);
 int gpufft_polymul_plan_get_radix_bits(gpufft_polymul_plan plan);
 /* Points per thread of the product kernel, 2^log2_points: 4 for
- * GPUFFT_PREC_F64 (512 threads per 2^13-point sub-problem); 5 (default, 512
+ * GPUFFT_PREC_F64 (512 threads per 2^13-point sub-problem); 5 (512
  * threads) or 4 (1024 threads) per 2^14-point sub-problem for
- * GPUFFT_PREC_F32_SPLIT.  Results do not depend on it beyond rounding.
+ * GPUFFT_PREC_F32_SPLIT (default 4 for ring_power 15 and 16, 5 otherwise).
+ * Results do not depend on it beyond rounding.
  * Fails with GPUFFT_ERR_UNSUPPORTED where no kernel exists for the plan's
  * size and radix (16 points per thread allow radix bits 3 and 4 only). */
 int gpufft_polymul_plan_set_thread_points(gpufft_polymul_plan plan, int log2_points);
