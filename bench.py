@@ -763,35 +763,37 @@ def main():
             utw = G.generate_table(G.TABLE_FFNT_TWIST, rp - 1, True, stream=sid)
             rt = G.generate_table(G.TABLE_FFNT_ROOT, rp - 1, False, stream=sid)
             irt = G.generate_table(G.TABLE_FFNT_ROOT, rp - 1, True, stream=sid)
-            io = torch.cat([a64, b64])
             temp = torch.empty(2 * batch * L, dtype=torch.complex128, device=dev)
             cf = G.config(rp, G.FORWARD, stream=sid)
             ci = G.config(rp, G.INVERSE, mod_inverse=1.0 / L, stream=sid)
 
-            def dropin():
+            def dropin(io):
                 G.GPU_FFNT(io, temp, tw, rt, cf, 2 * batch)
                 G.GPU_FFNT(io, temp, utw, irt, ci, batch, True)
 
-            ms = timed(lambda: (io[: batch * N].copy_(a64), dropin()), stream, args.steps,
-                       args.warmup)
-            ms_copy = timed(lambda: io[: batch * N].copy_(a64), stream, args.steps, 1)
-            ms_fwd = timed(lambda: G.GPU_FFNT(io, temp, tw, rt, cf, 2 * batch), stream, args.steps, 1)
-            io[: batch * N].copy_(a64)
-            dropin()
+            # the inverse writes the product over a, so every call gets its
+            # own [a; b] buffer, filled before the timed region (no restore
+            # copies inside it)
+            ios = [torch.cat([a64, b64]) for _ in range(args.steps + args.warmup)]
+            it = iter(ios)
+            ms = timed(lambda: dropin(next(it)), stream, args.steps, args.warmup)
+            ms_fwd = timed(lambda: G.GPU_FFNT(ios[0], temp, tw, rt, cf, 2 * batch), stream,
+                           args.steps, 1)
+            io = ios[-1]
             torch.cuda.synchronize()
             r = torch.round(io[: batch * N])
             d = (io[: batch * N] - r).abs()
+            del ios, it
             # the two-call API moves at least 56N bytes per product: 2 x 8N real
             # in and 16N spectra out (forward), 16N spectra in and 8N real out
             # (inverse with product); 24N is the product's own input and output
             api_bytes = 56 * N * batch
-            res["f64_dropin"] = {"ms": ms - ms_copy, "ms_forward": ms_fwd,
-                                 "ms_inverse": ms - ms_copy - ms_fwd,
+            res["f64_dropin"] = {"ms": ms, "ms_forward": ms_fwd,
+                                 "ms_inverse": ms - ms_fwd,
                                  "err_max": float(d.max()),
                                  "err_rms": float(torch.sqrt((d * d).mean())),
-                                 "api_io_GBps": api_bytes / ((ms - ms_copy) * 1e-3) / 1e9,
-                                 "hbm_frac_api_io": api_bytes / ((ms - ms_copy) * 1e-3) / 1e9 /
-                                 HBM_PEAK_GBS}
+                                 "api_io_GBps": api_bytes / (ms * 1e-3) / 1e9,
+                                 "hbm_frac_api_io": api_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
             outs["f64_dropin"] = r.to(torch.int64)
 
     # bit-exact agreement of the rounded products across forms
