@@ -502,9 +502,22 @@ def config_suite(args):
         exact_b = np.array_equal(gb[:2 * n - 1], np.convolve(a, b)) and not gb[2 * n - 1:].any()
         t_b = ev_ms(lambda: bp.graph_launch(da, db, bout, stream=sid), 50)
         bp.close()
+        # and as an FP64 batched plan (one workgroup item; a grid of one
+        # workgroup per XCD)
+        bp = G.BatchedPlan(descs, stream=sid, precision=G.PREC_F64)
+        bp.set_tuning(4, 8)
+        bout.zero_()
+        bp.graph_launch(da, db, bout, stream=sid)
+        torch.cuda.synchronize()
+        gb = bout.cpu().numpy()
+        exact_b64 = np.array_equal(gb[:2 * n - 1], np.convolve(a, b)) and not gb[2 * n - 1:].any()
+        t_b64 = ev_ms(lambda: bp.graph_launch(da, db, bout, stream=sid), 50)
+        bp.close()
         line = {"config": "C_C cyclic product, degree < 2^12, [0,16), zero-padded 2^13, f64",
                 "gpu_us_per_product": t_cc * 1e3, "bit_exact": bool(exact),
-                "batched_linear_fp32split_us": t_b * 1e3, "batched_linear_bit_exact": bool(exact_b)}
+                "batched_linear_fp32split_us": t_b * 1e3, "batched_linear_bit_exact": bool(exact_b),
+                "batched_linear_fp64_us": t_b64 * 1e3,
+                "batched_linear_fp64_bit_exact": bool(exact_b64)}
         if not args.no_cpu:
             import oracle as O
             ra = np.tile(a, (_host_threads(), 1))
