@@ -116,6 +116,8 @@ struct BArgs {
     int qlen[NQ];       // items of each queue
     int* next_item;     // NQ queue counters
     int* arrive;        // per product (in-kernel combine)
+    int* exited;        // workgroups that have left the persistent kernel
+    int count;          // products
     const int* split;   // the products with SB > 0 (combine kernel)
     int cmb;            // 1: split products are combined in the persistent kernel
     const int* a;
@@ -1132,6 +1134,17 @@ void batched_kernel(BArgs ba)
             __syncthreads();
         }
     }
+    // The last workgroup to leave clears the queue, arrival and exit
+    // counters for the next launch (every other workgroup has made its last
+    // atomic by then), so a launch needs no separate clearing kernel.
+    __syncthreads();
+    if (tid == 0) sh_last = atomicAdd(ba.exited, 1) == (int)gridDim.x - 1;
+    __syncthreads();
+    if (sh_last) {
+        for (int i = tid; i < NQ; i += BT) atomicExch(ba.next_item + i, 0);
+        for (int i = tid; i < ba.count; i += BT) atomicExch(ba.arrive + i, 0);
+        if (tid == 0) atomicExch(ba.exited, 0);
+    }
 }
 
 // One thread per pair of adjacent positions c, c + 1 < 2^S (S the tile) of
@@ -1189,7 +1202,7 @@ struct gpufft_batched_plan_s {
     BProd* d_prods;
     int2* d_items;
     BTables* d_tabs;
-    int* d_counters;  // [0, NQ) queue counters, [NQ, NQ + count) arrival counters
+    int* d_counters;  // [0, NQ) queue counters, [NQ, NQ + count) arrival counters, then the exit counter
     int* d_split;
     BSched sched;
     BSched* d_sched;
@@ -1389,9 +1402,7 @@ int launch(gpufft_batched_plan_s* p, const int* a, const int* b, long long* out,
            hipStream_t st)
 {
     const char* who = "gpufft_batched_execute";
-    hipLaunchKernelGGL(zero_ints, dim3((p->count + NQ + 255) / 256), dim3(256), 0, st,
-                       p->d_counters, p->count + NQ);
-    GFA_CHECK_LAUNCH(who);
+    // the counters are zero: cleared at plan creation and by the end of every launch
     if (err) {
         hipLaunchKernelGGL(zero_doubles, dim3((p->count + 255) / 256), dim3(256), 0, st, err, p->count);
         GFA_CHECK_LAUNCH(who);
@@ -1406,6 +1417,8 @@ int launch(gpufft_batched_plan_s* p, const int* a, const int* b, long long* out,
     }
     ba.next_item = p->d_counters;
     ba.arrive = p->d_counters + NQ;
+    ba.exited = p->d_counters + NQ + p->count;
+    ba.count = p->count;
     ba.split = p->d_split;
     ba.cmb = p->cmb;
     ba.a = a;
@@ -1568,9 +1581,16 @@ int create_plan(gpufft_batched_plan* out, const gpufft_product_desc* descs, int 
         release(p);
         return s;
     }
-    hipError_t e = hipMalloc((void**)&p->d_counters, sizeof(int) * (count + NQ));
+    hipError_t e = hipMalloc((void**)&p->d_counters, sizeof(int) * (count + NQ + 1));
     if (e == hipSuccess) e = hipMalloc(&p->wc, elem_bytes(p) * 32);
     if (e == hipSuccess) e = hipMalloc(&p->wci, elem_bytes(p) * 32);
+    if (e != hipSuccess) {
+        release(p);
+        return fail_hip(e, who);
+    }
+    hipLaunchKernelGGL(zero_ints, dim3((count + NQ + 1 + 255) / 256), dim3(256), 0, st,
+                       p->d_counters, count + NQ + 1);
+    e = hipGetLastError();
     if (e != hipSuccess) {
         release(p);
         return fail_hip(e, who);
