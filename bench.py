@@ -784,19 +784,26 @@ def main():
                 G.GPU_FFNT(io, temp, tw, rt, cf, 2 * batch)
                 G.GPU_FFNT(io, temp, utw, irt, ci, batch, True)
 
-            # the inverse writes the product over a, so every call gets its
-            # own [a; b] buffer, filled before the timed region (no restore
-            # copies inside it)
-            ios = [torch.cat([a64, b64]) for _ in range(args.steps + args.warmup)]
-            it = iter(ios)
-            ms = timed(lambda: dropin(next(it)), stream, args.steps, args.warmup)
+            # the inverse writes the product over a, so the calls cycle over
+            # up to 32 [a; b] buffers filled before the timed region (no
+            # restore copies inside it; a buffer used again holds an earlier
+            # product in its a half, the same work); the exactness check runs
+            # on a fresh buffer
+            ios = [torch.cat([a64, b64]) for _ in range(min(args.steps + args.warmup, 32))]
+            calls = [0]
+
+            def dropin_next():
+                dropin(ios[calls[0] % len(ios)])
+                calls[0] += 1
+            ms = timed(dropin_next, stream, args.steps, args.warmup)
             ms_fwd = timed(lambda: G.GPU_FFNT(ios[0], temp, tw, rt, cf, 2 * batch), stream,
                            args.steps, 1)
-            io = ios[-1]
+            del ios
+            io = torch.cat([a64, b64])
+            dropin(io)
             torch.cuda.synchronize()
             r = torch.round(io[: batch * N])
             d = (io[: batch * N] - r).abs()
-            del ios, it
             # the two-call API moves at least 56N bytes per product: 2 x 8N real
             # in and 16N spectra out (forward), 16N spectra in and 8N real out
             # (inverse with product); 24N is the product's own input and output
