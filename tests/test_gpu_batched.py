@@ -284,6 +284,31 @@ def test_batched_random_lists(trial):
         assert np.array_equal(to_host(out), exact), (tile, kinds, rps)
 
 
+@pytest.mark.parametrize("trial", range(4))
+def test_batched_f64_random_lists(trial):
+    """FP64 plans on seeded random lists (kinds, ring sizes up to NEGACYCLIC
+    2^18 / LINEAR 2^17, radix, grid, combine placement), each executed twice
+    with a retune between: exact against the NTT products every time."""
+    import torch
+
+    import gpufft_amd as G
+    rng = np.random.default_rng(0x6770756666740400 + trial)
+    cnt = int(rng.integers(3, 12))
+    kinds = [int(k) for k in rng.integers(0, 2, cnt)]
+    rps = [int(rng.integers(6 if k == 0 else 5, 19 if k == 0 else 18)) for k in kinds]
+    descs, a, b, exact = _workload(kinds, rps, 0x6770756666740500 + trial)
+    plan = G.BatchedPlan(descs, precision=G.PREC_F64)
+    da, db = to_dev(a.astype(np.int32)), to_dev(b.astype(np.int32))
+    for rep in range(2):
+        plan.set_tuning(int(rng.integers(3, 5)), int(rng.choice([8, 64, 256, 512])))
+        plan.combine_in_kernel = bool(rng.integers(0, 2))
+        out = torch.full((len(exact),), -5, dtype=torch.int64, device="cuda")
+        err = torch.zeros(len(descs), dtype=torch.float64, device="cuda")
+        plan.execute(da, db, out, err)
+        assert np.all(to_host(err) < 1e-3)
+        assert np.array_equal(to_host(out), exact), (rep, kinds, rps)
+
+
 def test_batched_autotune_db(tmp_path):
     """tune_batched times radix x grid x combine placement x tile, stores the
     winner in the tuning database, and a second plan gets it applied without
