@@ -129,6 +129,59 @@ def test_ffnt_forward_inverse(prec, n_power):
     assert rel_err(to_host(dr)[: batch * N], ref2) < TOL[prec]
 
 
+# sizes on the one-launch forward / two-launch inverse: 2^(S+1)..2^(S+3)
+# points, S = 13 (double) or 14 (float); R_R transforms 2^n_power points,
+# FFNT 2^(n_power-1)
+ONESHOT = [("f64", "r2r", 14), ("f64", "r2r", 16), ("f64", "ffnt", 15), ("f64", "ffnt", 17),
+           ("f32", "r2r", 15), ("f32", "r2r", 17), ("f32", "ffnt", 16), ("f32", "ffnt", 18)]
+
+
+@pytest.mark.parametrize("prec,kind,n_power", ONESHOT)
+def test_oneshot_odd_batches(prec, kind, n_power):
+    """Batches of 13 and 21 transforms on the one-launch paths: their grids
+    deal the workgroups of a transform 8 transforms at a time (block ids
+    congruent mod 8), so a batch that is not a multiple of 8 leaves a partial
+    group, and one above 8 spans several; forward and inverse with product
+    against the oracle."""
+    G = _g()
+    import torch
+    rng = np.random.default_rng(4000 + n_power * 4 + (kind == "ffnt") * 2 + (prec == "f32"))
+    for batch in (13, 21):
+        if kind == "r2r":
+            N = L = 1 << n_power
+            tabs = _tables(n_power, 0, prec)
+            ftab, itab = tabs["rev"], tabs["inv_rev"]
+        else:
+            N = 1 << n_power
+            L = N // 2
+            t = O.ffnt_tables(N, prec)
+            ftab, itab = t["rev_root"], t["inv_rev_root"]
+        real = rng.standard_normal(2 * batch * N).astype(rdtype(prec))
+        dr = to_dev(real)
+        dt = torch.zeros(2 * batch * L, dtype=torch.complex128 if prec == "f64" else torch.complex64,
+                         device="cuda")
+        if kind == "r2r":
+            G.GPU_FFT_R(dr, dt, to_dev(ftab), G.config(n_power, G.FORWARD, G.X_N_MINUS), 2 * batch)
+            ref = O.r2r_forward(real, n_power, 2 * batch, ftab, 0, prec)
+        else:
+            G.GPU_FFNT(dr, dt, to_dev(t["twist"]), to_dev(ftab), G.config(n_power, G.FORWARD),
+                       2 * batch)
+            ref = O.ffnt_forward(real, n_power, 2 * batch, t["twist"], ftab, prec)
+        assert rel_err(to_host(dt), ref) < TOL[prec], batch
+        dt2 = to_dev(ref)
+        if kind == "r2r":
+            G.GPU_FFT_R(dr, dt2, to_dev(itab), G.config(n_power, G.INVERSE, G.X_N_MINUS, 1.0 / L),
+                        batch, True)
+            ref2 = O.r2r_inverse(ref, n_power, batch, itab, 0, 1, complex(1.0 / L), prec)
+        else:
+            G.GPU_FFNT(dr, dt2, to_dev(t["untwist"]), to_dev(itab),
+                       G.config(n_power, G.INVERSE, mod_inverse=1.0 / L), batch, True)
+            ref2 = O.ffnt_inverse(ref, n_power, batch, t["untwist"], itab, 1, complex(1.0 / L), prec)
+        assert rel_err(to_host(dr)[: batch * N], ref2) < TOL[prec], batch
+        # the second half of the real buffer (b) is left alone by the inverse
+        assert np.array_equal(to_host(dr)[batch * N:], real[batch * N:]), batch
+
+
 @pytest.mark.parametrize("prec", PRECS)
 @pytest.mark.parametrize("n_power", [1, 2, 5, 11, 12, 13, 14, 15, 17])
 def test_special_forward_inverse(prec, n_power):
