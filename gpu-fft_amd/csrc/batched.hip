@@ -1147,8 +1147,19 @@ void batched_kernel(BArgs ba)
     }
 }
 
-// One thread per pair of adjacent positions c, c + 1 < 2^S (S the tile) of
-// one split product (blockIdx.x: the product's index in the split list).
+// Positions cbase + t and cbase + CT + t of one split product (t the
+// thread, cbase the workgroup's first of 2 CT positions).
+template <typename T, int SB, bool UL>
+__device__ __forceinline__ float combine_two(const BArgs& ba, const BProd& pr, const BTables& tb,
+                                             const cplx<T>* uhi, const cplx<T>* ulo, int cbase)
+{
+    const int c = cbase + threadIdx.x;
+    return fmaxf(combine_pos<T, SB, UL>(ba, pr, tb, c, uhi, ulo, cbase),
+                 combine_pos<T, SB, UL>(ba, pr, tb, c + CT, uhi, ulo, cbase));
+}
+
+// The 2 CT positions from cbase < 2^S (S the tile) of one split product
+// (blockIdx.x: the product's index in the split list, blockIdx.y: cbase).
 // FP64: the untwist factors come from LDS, untwist[P] = untwist[P & ~127]
 // untwist[P & 127] (exp(-2 pi i P / 2N) is multiplicative in P; the FP32
 // form keeps the direct table entries, whose rounding its limb error
@@ -1171,14 +1182,28 @@ template <typename T> __global__ __launch_bounds__(CT) void batched_combine_kern
         __syncthreads();
     }
     float em;
-    switch (pr.SB) {
-        case 1: em = combine_pos2<T, 1, UL>(ba, pr, tb, c, uhi, ulo, cbase); break;
-        case 2: em = combine_pos2<T, 2, UL>(ba, pr, tb, c, uhi, ulo, cbase); break;
-        case 3: em = combine_pos2<T, 3, UL>(ba, pr, tb, c, uhi, ulo, cbase); break;
-        default:  // 2^4 rows: one position at a time (registers), consecutive per lane
-            em = fmaxf(combine_pos<T, 4, UL>(ba, pr, tb, cbase + threadIdx.x, uhi, ulo, cbase),
-                       combine_pos<T, 4, UL>(ba, pr, tb, cbase + CT + threadIdx.x, uhi, ulo, cbase));
-            break;
+    if constexpr (sizeof(T) == 8) {
+        // FP64: positions cbase + t and cbase + CT + t, so that the lanes of
+        // every 16-byte scratch load and 8-byte output store touch
+        // consecutive elements (pairs of adjacent positions per thread put
+        // the lanes of each load 32 bytes apart)
+        switch (pr.SB) {
+            case 1: em = combine_two<T, 1, UL>(ba, pr, tb, uhi, ulo, cbase); break;
+            case 2: em = combine_two<T, 2, UL>(ba, pr, tb, uhi, ulo, cbase); break;
+            case 3: em = combine_two<T, 3, UL>(ba, pr, tb, uhi, ulo, cbase); break;
+            default: em = combine_two<T, 4, UL>(ba, pr, tb, uhi, ulo, cbase); break;
+        }
+    } else {
+        // FP32: a pair of adjacent positions is one 16-byte load per row and
+        // one 16-byte store per output pair, consecutive across the lanes
+        switch (pr.SB) {
+            case 1: em = combine_pos2<T, 1, UL>(ba, pr, tb, c, uhi, ulo, cbase); break;
+            case 2: em = combine_pos2<T, 2, UL>(ba, pr, tb, c, uhi, ulo, cbase); break;
+            case 3: em = combine_pos2<T, 3, UL>(ba, pr, tb, c, uhi, ulo, cbase); break;
+            default:  // 2^4 rows: one position at a time (registers), consecutive per lane
+                em = combine_two<T, 4, UL>(ba, pr, tb, uhi, ulo, cbase);
+                break;
+        }
     }
     err_update(ba.err, prod, em);
 }

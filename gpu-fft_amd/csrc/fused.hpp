@@ -575,10 +575,12 @@ __global__ __launch_bounds__((1 << S) >> E) void product_sub_kernel(ProdArgs<T> 
 // squares over all limb outputs.
 // FP64 products: top SB Gentleman-Sande stages (bits S .. n-1), scale,
 // untwist and unfold (reference InverseFFNTCoreUnique, src/lib/fft.cu:
-// 1298-1384), one thread per pair of adjacent columns c, c+1 of one product:
-// 16-byte loads of the 2^SB rows, 16-byte stores of the two real and the two
-// imaginary outputs, untwist[P] = untwist[P & ~127] * untwist[P & 127] from
-// LDS (the table is exp(-2 pi i P / 2N), multiplicative in P).
+// 1298-1384).  A workgroup takes 512 consecutive columns of one product
+// (S >= 9), each thread the columns c and c + 256: the lanes of every load
+// read consecutive 16-byte scratch entries and the lanes of every store
+// write consecutive doubles.  untwist[P] = untwist[P & ~127] *
+// untwist[P & 127] from LDS (the table is exp(-2 pi i P / 2N),
+// multiplicative in P).
 struct F64CombineArgs {
     int n, S, batch;
     const cplx<double>* scratch;  // [batch][2^n]
@@ -601,9 +603,9 @@ __global__ __launch_bounds__(256) void f64_combine_kernel(F64CombineArgs a)
     for (int i = tid; i < 128 + nhi; i += 256)
         ut[i] = i < 128 ? a.untwist[i] : a.untwist[(i - 128) << 7];
     __syncthreads();
-    const long long gid = (long long)blockIdx.x * 256 + tid;
-    const long long p = gid >> (a.S - 1);
-    const int c = (int)(gid & ((1LL << (a.S - 1)) - 1)) * 2;
+    const long long g0 = (long long)blockIdx.x * 512;
+    const long long p = g0 >> a.S;
+    const int c = (int)(g0 & ((1LL << a.S) - 1)) + tid;
     if (p >= a.batch) return;
     const long long L = 1LL << a.n;
     const cplx<double>* src = a.scratch + p * L;
@@ -614,7 +616,7 @@ __global__ __launch_bounds__(256) void f64_combine_kernel(F64CombineArgs a)
 #pragma unroll
     for (int j = 0; j < Q; j++) {
         y[0][j] = ldc_nt(src + ((long long)j << a.S) + c);
-        y[1][j] = ldc_nt(src + ((long long)j << a.S) + c + 1);
+        y[1][j] = ldc_nt(src + ((long long)j << a.S) + c + 256);
     }
     double* o = a.out + p * a.out_stride;
 #pragma unroll
@@ -633,14 +635,14 @@ __global__ __launch_bounds__(256) void f64_combine_kernel(F64CombineArgs a)
     }
 #pragma unroll
     for (int j = 0; j < Q; j++) {
-        const int P = (j << a.S) | c;
-        cplx<double> z[2];
 #pragma unroll
-        for (int v = 0; v < 2; v++)
-            z[v] = mk<double>(y[v][j].x * a.scale, y[v][j].y * a.scale) *
-                   (ut[128 + ((P + v) >> 7)] * ut[(P + v) & 127]);
-        stc_nt(reinterpret_cast<cplx<double>*>(o + P), mk<double>(z[0].x, z[1].x));
-        stc_nt(reinterpret_cast<cplx<double>*>(o + P + L), mk<double>(z[0].y, z[1].y));
+        for (int v = 0; v < 2; v++) {
+            const int P = (j << a.S) | (c + 256 * v);
+            const cplx<double> z = mk<double>(y[v][j].x * a.scale, y[v][j].y * a.scale) *
+                                   (ut[128 + (P >> 7)] * ut[P & 127]);
+            __builtin_nontemporal_store(z.x, o + P);
+            __builtin_nontemporal_store(z.y, o + P + L);
+        }
     }
 }
 

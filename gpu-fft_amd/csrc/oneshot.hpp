@@ -174,14 +174,15 @@ __global__ __launch_bounds__((1 << S) >> E) void inv_local_kernel(OneArgs<T> o)
 }
 
 // Inverse, second launch (SB = 2..4): the top SB Gentleman-Sande stages (bits S .. n-1)
-// of two adjacent columns c, c+1 per thread from the 2^SB rows j 2^S + c of
-// the first launch's result, twiddle tab[base_b + (j0 >> (i+1))] (uniform:
-// the column bits are below the stage bit), then the scale and the output:
-// ST_REAL Re(z mod_inverse) (reference InverseCore R_R store,
-// fft.cu:499-514), ST_UNFOLD z mod_inverse untwist[p] unfolded to Re at p
-// and Im at p + L (InverseFFNTCoreUnique, fft.cu:1366-1383), with
-// untwist[p] = untwist[p & ~127] untwist[p & 127] from LDS.  Two adjacent
-// outputs per store.
+// from the 2^SB rows j 2^S + c of the first launch's result, twiddle
+// tab[base_b + (j0 >> (i+1))] (uniform: the column bits are below the stage
+// bit), then the scale and the output: ST_REAL Re(z mod_inverse) (reference
+// InverseCore R_R store, fft.cu:499-514), ST_UNFOLD z mod_inverse untwist[p]
+// unfolded to Re at p and Im at p + L (InverseFFNTCoreUnique,
+// fft.cu:1366-1383), with untwist[p] = untwist[p & ~127] untwist[p & 127]
+// from LDS.  A workgroup takes 512 consecutive columns of one transform
+// (S >= 9), each thread the columns c and c + 256, so that the lanes of
+// every load and every store touch consecutive elements.
 template <typename T, int SB, int SK>
 __global__ __launch_bounds__(256) void inv_top_kernel(OneArgs<T> o, int S)
 {
@@ -196,9 +197,9 @@ __global__ __launch_bounds__(256) void inv_top_kernel(OneArgs<T> o, int S)
             ut[i] = i < nhi ? o.untwist[i << 7] : o.untwist[i - nhi];
         __syncthreads();
     }
-    const long long gid = (long long)blockIdx.x * 256 + tid;
-    const long long t = gid >> (S - 1);
-    const int c = (int)(gid & ((1LL << (S - 1)) - 1)) * 2;
+    const long long g0 = (long long)blockIdx.x * 512;
+    const long long t = g0 >> S;
+    const int c = (int)(g0 & ((1LL << S) - 1)) + tid;
     if (t >= o.batch) return;
     const long long L = 1LL << n;
     const cplx<T>* src = reinterpret_cast<const cplx<T>*>(o.src) + t * o.src_stride;
@@ -214,7 +215,7 @@ __global__ __launch_bounds__(256) void inv_top_kernel(OneArgs<T> o, int S)
 #pragma unroll
     for (int j = 0; j < Q; j++) {
         y[0][j] = ldc_nt(src + ((long long)j << S) + c);
-        y[1][j] = ldc_nt(src + ((long long)j << S) + c + 1);
+        y[1][j] = ldc_nt(src + ((long long)j << S) + c + 256);
     }
 #pragma unroll
     for (int v = 0; v < 2; v++) {
@@ -231,26 +232,15 @@ __global__ __launch_bounds__(256) void inv_top_kernel(OneArgs<T> o, int S)
         }
     }
     T* out = reinterpret_cast<T*>(o.dst) + t * o.dst_stride;
-    typedef T vec2 __attribute__((ext_vector_type(2)));
 #pragma unroll
     for (int j = 0; j < Q; j++) {
-        const int P = (j << S) | c;
-        cplx<T> z[2];
 #pragma unroll
         for (int v = 0; v < 2; v++) {
-            z[v] = y[v][j] * o.scale;
-            if constexpr (SK == ST_UNFOLD)
-                z[v] = z[v] * (ut[(P + v) >> 7] * ut[(1 << (n - 7)) + ((P + v) & 127)]);
-        }
-        vec2 re;
-        re.x = z[0].x;
-        re.y = z[1].x;
-        __builtin_nontemporal_store(re, reinterpret_cast<vec2*>(out + P));
-        if constexpr (SK == ST_UNFOLD) {
-            vec2 im;
-            im.x = z[0].y;
-            im.y = z[1].y;
-            __builtin_nontemporal_store(im, reinterpret_cast<vec2*>(out + P + L));
+            const int P = (j << S) | (c + 256 * v);
+            cplx<T> z = y[v][j] * o.scale;
+            if constexpr (SK == ST_UNFOLD) z = z * (ut[P >> 7] * ut[(1 << (n - 7)) + (P & 127)]);
+            __builtin_nontemporal_store(z.x, out + P);
+            if constexpr (SK == ST_UNFOLD) __builtin_nontemporal_store(z.y, out + P + L);
         }
     }
 }

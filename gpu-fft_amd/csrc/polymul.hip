@@ -264,7 +264,9 @@ int run_f64(const gpufft_polymul_plan_s* plan, const double* a, const double* b,
             ca.scale = 1.0 / (double)(1LL << plan->n);
             ca.out = po;
             ca.out_stride = N;
-            const unsigned blocks = (unsigned)((((long long)nb << (plan->S - 1)) + 255) / 256);
+            // 512 columns per workgroup, every workgroup inside one product
+            if (plan->S < 9) return fail(GPUFFT_ERR_UNSUPPORTED, std::string(who) + ": tile too small");
+            const unsigned blocks = (unsigned)((long long)nb << (plan->S - 9));
             switch (plan->SB) {
                 case 1: hipLaunchKernelGGL(f64_combine_kernel<1>, dim3(blocks), dim3(256), 0, st, ca); break;
                 case 2: hipLaunchKernelGGL(f64_combine_kernel<2>, dim3(blocks), dim3(256), 0, st, ca); break;

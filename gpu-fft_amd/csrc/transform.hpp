@@ -257,7 +257,8 @@ bool run_oneshot(int mode, int n, int batch, int nega, const cplx<T>* tab,
         constexpr int SI = OneCfg<T>::SI, EI = OneCfg<T>::EI;
         const int SBI = n - SI;  // 2..4
         const long long blocks1 = (long long)batch << SBI;
-        const long long blocks2 = (((long long)batch << (SI - 1)) + 255) / 256;
+        static_assert(SI >= 9, "the second inverse launch takes 512 columns per workgroup");
+        const long long blocks2 = (long long)batch << (SI - 9);
         if (blocks1 > 0x7fffffffLL || blocks2 > 0x7fffffffLL) {
             *status = fail(GPUFFT_ERR_INVALID_ARGUMENT, std::string(who) + ": batch too large");
             return true;
