@@ -64,9 +64,11 @@ def parse():
 
 
 def calibrate_copy():
-    """Three plain device-to-device copies of 512 MiB: 536870912 bytes read and
+    """Plain device-to-device copies of 512 MiB: 536870912 bytes read and
     536870912 bytes written per copy, the known amounts against which the HBM
-    traffic counters (FETCH_SIZE, WRITE_SIZE) are scaled."""
+    traffic counters (FETCH_SIZE, WRITE_SIZE) are scaled.  Three untimed
+    copies, then ten timed ones: the streaming rate of a plain copy (twice
+    the 256 MB last-level cache, so it runs from HBM)."""
     import torch
     n = 64 << 20
     x = torch.ones(n, dtype=torch.float64, device="cuda")
@@ -74,8 +76,11 @@ def calibrate_copy():
     for _ in range(3):
         y.copy_(x)
     torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ms = timed(lambda: y.copy_(x), stream, 10, 0)
     print(json.dumps({"calibration": "copy", "bytes_read_per_copy": 8 * n,
-                      "bytes_written_per_copy": 8 * n, "copies": 3}))
+                      "bytes_written_per_copy": 8 * n, "copies": 13, "ms_per_copy": ms,
+                      "GBps": 2 * 8 * n / (ms * 1e-3) / 1e9}))
 
 
 def timed(fn, stream, steps, warmup):
