@@ -531,11 +531,29 @@ def config_suite(args):
         exact_b64 = np.array_equal(gb[:2 * n - 1], np.convolve(a, b)) and not gb[2 * n - 1:].any()
         t_b64 = ev_ms(lambda: bp.graph_launch(da, db, bout, stream=sid), 50)
         bp.close()
+        # and as the fused FP64 product mod X^(2^13) + 1: operands of degree
+        # < 2^12 keep the product below X^(2^13), so the negacyclic product
+        # is the linear one (one workgroup, transforms of 2^12 complex
+        # points), graph-replayed
+        fp = G.PolymulPlan(13, G.PREC_F64, max_batch=1, stream=sid)
+        fa = torch.zeros(L, dtype=torch.float64, device=dev)
+        fb = torch.zeros(L, dtype=torch.float64, device=dev)
+        fa[:n] = torch.from_numpy(a.astype(np.float64))
+        fb[:n] = torch.from_numpy(b.astype(np.float64))
+        fo = torch.empty(L, dtype=torch.float64, device=dev)
+        fp.multiply_f64(fa, fb, fo, 1, stream=sid)
+        torch.cuda.synchronize()
+        gf = np.round(fo.cpu().numpy()).astype(np.int64)
+        exact_f = np.array_equal(gf[:2 * n - 1], np.convolve(a, b)) and not gf[2 * n - 1:].any()
+        t_f = graph_ms(lambda: fp.multiply_f64(fa, fb, fo, 1, stream=sid))
+        fp.close()
         line = {"config": "C_C cyclic product, degree < 2^12, [0,16), zero-padded 2^13, f64",
                 "gpu_us_per_product": t_cc * 1e3, "bit_exact": bool(exact),
                 "batched_linear_fp32split_us": t_b * 1e3, "batched_linear_bit_exact": bool(exact_b),
                 "batched_linear_fp64_us": t_b64 * 1e3,
-                "batched_linear_fp64_bit_exact": bool(exact_b64)}
+                "batched_linear_fp64_bit_exact": bool(exact_b64),
+                "fused_negacyclic_2p13_fp64_us": t_f * 1e3,
+                "fused_negacyclic_2p13_fp64_bit_exact": bool(exact_f)}
         if not args.no_cpu:
             import oracle as O
             ra = np.tile(a, (_host_threads(), 1))

@@ -84,6 +84,29 @@ def test_polymul_f64_exact(ring_power, combine, rb):
     assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
 
 
+def test_polymul_f64_linear_through_negacyclic():
+    """A linear product whose degree stays below the ring (operands of degree
+    < 2^12 in the ring mod X^(2^13) + 1, the C1 shape of the reference's
+    example C_C product) is the negacyclic one: the fused FP64 product
+    equals the schoolbook convolution, with zeros above degree 2^13 - 2."""
+    G = _g()
+    import torch
+    n, N = 1 << 12, 1 << 13
+    a = O.fill_uniform(SEED_BASE + 91, 0, 16, n)
+    b = O.fill_uniform(SEED_BASE + 92, 0, 16, n)
+    pa = np.zeros(N, np.float64)
+    pb = np.zeros(N, np.float64)
+    pa[:n], pb[:n] = a, b
+    plan = G.PolymulPlan(13, G.PREC_F64, max_batch=1)
+    out = torch.empty(N, dtype=torch.float64, device="cuda")
+    plan.multiply_f64(to_dev(pa), to_dev(pb), out, 1)
+    got = np.round(to_host(out)).astype(np.int64)
+    plan.close()
+    ref = np.convolve(a.astype(np.int64), b.astype(np.int64))
+    assert np.array_equal(got[:2 * n - 1], ref)
+    assert not got[2 * n - 1:].any()
+
+
 @pytest.mark.parametrize("in_kernel", [True, False])
 def test_polymul_f64_chunks_and_reuse(in_kernel):
     """batch larger than max_batch runs in chunks; a second call on the same
