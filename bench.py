@@ -591,12 +591,29 @@ def config_suite(args):
         exact = _poly_check(a, b, prod)
         fwd_bytes = 2 * B * (N * 8 + N * 16)   # real in, full complex spectrum out
         mul_bytes = B * 3 * N * 8              # two real operands in, real product out
+        # and as the fused FP64 product mod X^(2^16) + 1: operands of degree
+        # < 2^15 keep the product below X^(2^16), where the cyclic, the
+        # negacyclic and the linear products coincide
+        fp = G.PolymulPlan(16, G.PREC_F64, max_batch=B, stream=sid)
+        fo = torch.empty(B * N, dtype=torch.float64, device=dev)
+        t_fused = ev_ms(lambda: fp.multiply_f64(rsrc[:B * N], rsrc[B * N:], fo, B, stream=sid),
+                        args.steps)
+        fp.multiply_f64(rsrc[:B * N], rsrc[B * N:], fo, B, stream=sid)
+        torch.cuda.synchronize()
+        prod_f = np.round(fo.cpu().numpy().reshape(B, N)).astype(np.int64)
+        exact_f = _poly_check(a, b, prod_f) and bool(np.array_equal(prod_f, prod))
+        fp.close()
+        del fo
         line = {"config": "R_R cyclic products, N=2^16 (degree < 2^15), batch 64, "
                           "[-2^10,2^10), f64",
                 "forward_ms": t_fwd, "forward_alg_GBps": fwd_bytes / (t_fwd * 1e-3) / 1e9,
                 "multiply_ms": t_mul, "multiply_alg_GBps": mul_bytes / (t_mul * 1e-3) / 1e9,
                 "products_per_s": B / (t_mul * 1e-3),
-                "bit_exact_at_random_points_mod_2^31-1": exact}
+                "bit_exact_at_random_points_mod_2^31-1": exact,
+                "fused_negacyclic_fp64_ms": t_fused,
+                "fused_negacyclic_alg_GBps": mul_bytes / (t_fused * 1e-3) / 1e9,
+                "fused_negacyclic_products_per_s": B / (t_fused * 1e-3),
+                "fused_negacyclic_bit_exact": exact_f}
         if not args.no_cpu:
             import oracle as O
             line.update(_cpu_rates(lambda th, m: O.cpu_fft_mult(a[:m], b[:m], threads=th),
